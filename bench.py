@@ -1,0 +1,194 @@
+"""Headline benchmark: site-updates/s of arrested Newton flow, pion Skyrme, 128^3, B=4.
+
+One *step* = one full ANF step of the whole lattice (4 RK4 gradient stages,
+RK4 combine, S^3 renormalisation / tangent projection, energy, baryon-number
+and max-force reductions); one *site-update* = one site advanced one step.
+
+Workload (one GPU): pion-only Skyrme model, 128^3 sites, dx = 0.15, m = 1,
+kappa = 1, dt = 0.5 dx, B = 4 rational map R(z) = (z^4 + 2 sqrt(3) i z^2 + 1) /
+(z^4 - 2 sqrt(3) i z^2 + 1) with profile f(r) = pi exp(-r/1.5), N(0, 1e-3)
+noise then renormalised, vacuum boundary band.  Fields are resident in HBM
+before the timed region.  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from skyrmion_solver_amd import _engine, initial_data  # noqa: E402
+from skyrmion_solver_amd.theories import load_theory  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "site-updates/s at 128^3 B=4 pion ANF, FP32 vs FP64; p50/p99 solve latency"
+# compulsory HBM traffic of each kernel in nf-vectors of the storage type:
+# stage 1 reads F, writes l1; 2 reads F, V, writes l2; 3 reads F, V, l1, writes l3;
+# 4 reads F, V, l1, l2, l3, writes F', V'; energy pass reads F
+STAGE_VECTORS = {0: 1, 1: 2, 2: 3, 3: 4, 4: 7}
+STAGE_NAMES = {0: "energy", 1: "stage1", 2: "stage2", 3: "stage3", 4: "stage4"}
+
+
+def workload(L: int = 128, dx: float = 0.15, seed: int = 0):
+    theory = load_theory("Nuclear Skyrme model")
+    p = theory.params.default_params(xlen=L, ylen=L, zlen=L, xsize=dx * (L - 1), ysize=dx * (L - 1),
+                                     zsize=dx * (L - 1), lsx=dx, lsy=dx, lsz=dx, courant=0.5, mpi=1.0, kappa=1.0,
+                                     baryon_number=4.0, ansatz="rational_map")
+    rp = p.resolved()
+    p_i, p_f = theory.params.pack_device_params(rp)
+    cfg = dict(mode="rational_map", baryon_number=4, profile=("exp", 1.5), noise=1.0e-3, seed=seed, vacuum_band=True)
+    F = initial_data.build_field(rp, 4, cfg)
+    return rp, p_i, p_f, F
+
+
+def time_form(prec, p_i, p_f, F, steps, warmup, chunk):
+    eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
+    eng.upload(_engine.FIELD, F)
+    eng.anf_run(warmup, -1.0, chunk)
+    eng.sync()
+    t0 = time.perf_counter()
+    st = eng.anf_run(steps, -1.0, chunk)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    out = dict(seconds=dt, steps=st.steps, energy=st.energy, err_abs=st.err_abs, baryon=st.baryon)
+    eng.close()
+    return out
+
+
+def trace_form(prec, p_i, p_f, F, steps):
+    """Per-kernel durations from HIP events on the engine's stream (direct launches)."""
+    eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f, flags=_engine.FLAG_TRACE)
+    eng.upload(_engine.FIELD, F)
+    eng.anf_run(3, -1.0, 3)
+    eng.trace_reset()
+    eng.anf_run(steps, -1.0, steps)
+    ms, cnt = eng.trace_read()
+    eng.close()
+    return {STAGE_NAMES[k]: dict(avg_us=1000.0 * ms[k] / cnt[k], launches=int(cnt[k])) for k in range(5) if cnt[k]}
+
+
+def solve_latency(prec, p_i, p_f, seeds, tol, max_steps):
+    lat, steps = [], []
+    for s in seeds:
+        _, _, _, F = workload(seed=s)
+        eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
+        eng.upload(_engine.FIELD, F)
+        eng.sync()
+        t0 = time.perf_counter()
+        st = eng.anf_run(max_steps, tol, 64)
+        eng.sync()
+        lat.append(time.perf_counter() - t0)
+        steps.append(st.steps)
+        eng.close()
+    lat = np.array(lat)
+    return dict(tol=tol, seeds=len(seeds), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                steps=steps)
+
+
+def cpu_baseline(p_i, p_f, F, steps: int = 1):
+    """FP64 NumPy oracle (single-threaded) on the host: `steps` ANF steps of the same 128^3 workload."""
+    from oracle import integrator as OI
+
+    st = OI.OracleState("pion", p_i, p_f, F)
+    E = OI.energy(st)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E, _ = OI.anf_step(st, E)
+    dt = time.perf_counter() - t0
+    N = int(p_i[6])
+    return dict(value=N * steps / dt, unit="site-updates/s", cores=1, kind="port",
+                sample=f"{steps} ANF step(s) of the 128^3 B=4 pion workload, FP64 NumPy oracle, 1 thread, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--trace-steps", type=int, default=20)
+    ap.add_argument("--solve-seeds", type=int, default=3)
+    ap.add_argument("--solve-max-steps", type=int, default=20000)
+    ap.add_argument("--tol", type=float, default=1.0e-4)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+
+    rp, p_i, p_f, F = workload()
+    N = int(p_i[6])
+    nf = 4
+    res = {}
+    for name, prec in (("fp64", _engine.FP64), ("fp32", _engine.FP32)):
+        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk)
+        r["site_updates_per_s"] = N * r["steps"] / r["seconds"]
+        r["ms_per_step"] = 1000.0 * r["seconds"] / r["steps"]
+        s = 8 if prec == _engine.FP64 else 4
+        r["algorithmic_GBs"] = r["site_updates_per_s"] * 16 * nf * s / 1e9
+        r["kernels"] = trace_form(prec, p_i, p_f, F, args.trace_steps)
+        res[name] = r
+        print(f"[bench] {name}: {r['site_updates_per_s']:.4g} site-updates/s, {r['ms_per_step']:.4f} ms/step, "
+              f"{r['algorithmic_GBs']:.1f} GB/s algorithmic", file=sys.stderr, flush=True)
+
+    head = res["fp32"]
+    # dominant kernel of the headline form: largest total time per step
+    kern = head["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * (1 if k != "energy" else 0))
+    stage = [k for k, v in STAGE_NAMES.items() if v == dom][0]
+    bytes_per_launch = STAGE_VECTORS[stage] * nf * 4 * N
+    achieved = bytes_per_launch / (kern[dom]["avg_us"] * 1e-6) / 1e9
+    roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    bytes_per_launch=bytes_per_launch)
+
+    latency = {}
+    if args.solve_seeds > 0:
+        for name, prec in (("fp32", _engine.FP32), ("fp64", _engine.FP64)):
+            latency[name] = solve_latency(prec, p_i, p_f, list(range(args.solve_seeds)), args.tol,
+                                          args.solve_max_steps)
+            print(f"[bench] solve {name}: {latency[name]}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p_i, p_f, F, args.cpu_steps)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": head["site_updates_per_s"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pion Skyrme ANF, 128^3, dx=0.15, m=1, kappa=1, dt=0.075, B=4 rational map "
+                               "f=pi*exp(-r/1.5), noise 1e-3, FP32 storage with FP64 reductions",
+                   "sites": N, "components": nf},
+        "fp64": {"value": res["fp64"]["site_updates_per_s"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "algorithmic_GBs": res["fp64"]["algorithmic_GBs"], "kernels": res["fp64"]["kernels"]},
+        "fp32": {"algorithmic_GBs": head["algorithmic_GBs"], "kernels": head["kernels"]},
+        "fp32_over_fp64": head["site_updates_per_s"] / res["fp64"]["site_updates_per_s"],
+        "solve_latency": latency,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * skx.h -- C ABI of the MI355X (gfx950) engine for the skyrmion_solver hot path.
+ *
+ * The engine replaces the per-step Numba-CUDA kernels of the reference
+ * (Paulnleask/skyrmion_solver) behind its Python `Simulation` API.  Each entry
+ * point below names the reference interface it replaces (file:line relative
+ * to the reference's `skyrmion_solver/` package).
+ *
+ * Conventions
+ *   - Return 0 (SKX_OK) on success, a negative SKX_E* code on failure;
+ *     skx_last_error() then holds a message (thread-local).  Nothing aborts.
+ *   - Host arrays are always FP64 in the reference layout
+ *     idx = z + Z*(y + Y*x) + N*a  (core/utils.py:48); the engine converts to
+ *     its storage precision.  Host pointers are borrowed for the call only.
+ *   - A handle owns all of its device memory and one HIP stream.  A handle is
+ *     not re-entrant; distinct handles may be driven from different threads.
+ */
+#ifndef SKX_H
+#define SKX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKX_VERSION 1
+
+enum skx_status_code {
+  SKX_OK = 0,
+  SKX_EINVAL = -1,      /* invalid argument / parameter array */
+  SKX_ENOMEM = -2,      /* device allocation failed */
+  SKX_EHIP = -3,        /* HIP runtime error (message in skx_last_error) */
+  SKX_ENONFINITE = -4,  /* a non-finite energy or gradient was produced */
+  SKX_EBUSY = -5,       /* resource busy (admission refused) */
+  SKX_ENOTSUP = -6      /* theory / option not supported by this build */
+};
+
+/* Theories: the plugins of theories/registry.py that the engine accelerates. */
+enum skx_theory {
+  SKX_THEORY_PION = 0,    /* theories/nuclear_skyrme  ("Nuclear Skyrme model") */
+  SKX_THEORY_OMEGA = 1,   /* theories/omega_skyrme    ("Omega-Skyrme model")   */
+  SKX_THEORY_COULOMB = 2, /* theories/coulomb_skyrme  ("Coulomb-Skyrme model") */
+  SKX_THEORY_RHO = 3      /* theories/rho_skyrme      ("Rho-Skyrme model")     */
+};
+
+enum skx_precision {
+  SKX_FP64 = 0, /* plain form: FP64 storage and arithmetic (reference arithmetic) */
+  SKX_FP32 = 1  /* FP32 storage and stencils; FP64 reductions, RK4 combine and S^3 renormalisation */
+};
+
+enum skx_buffer { SKX_FIELD = 0, SKX_VELOCITY = 1 };
+
+enum skx_density_kind {
+  SKX_DENSITY_ENERGY = 0,    /* compute_energy_kernel (theories/nuclear_skyrme/kernels.py:349-382) */
+  SKX_DENSITY_BARYON = 1,    /* compute_skyrmion_number_kernel (:449-482) */
+  SKX_DENSITY_POTENTIAL = 2, /* compute_potential_kernel (theories/omega_skyrme/kernels.py:412-441) */
+  SKX_DENSITY_RHO = 3        /* compute_rho_meson_kernel (theories/rho_skyrme/kernels.py:384-412) */
+};
+
+enum skx_flags {
+  SKX_FLAG_NO_GRAPH = 1, /* launch stage kernels directly instead of replaying a captured hipGraph */
+  SKX_FLAG_TRACE = 2     /* time every stage kernel with HIP events */
+};
+
+/* Lattice + theory description.  p_i / p_f are the reference's packed
+ * parameter arrays verbatim (core/params.py:248-268 plus the theory's
+ * pack_device_params, e.g. theories/nuclear_skyrme/params.py:203-245). */
+typedef struct skx_desc {
+  int32_t theory;       /* enum skx_theory */
+  int32_t precision;    /* enum skx_precision */
+  const int32_t* p_i;
+  int32_t n_i;
+  const double* p_f;
+  int32_t n_f;
+  int32_t device;       /* HIP device ordinal */
+  uint32_t flags;       /* enum skx_flags */
+  int32_t tile_z;       /* 0 = engine default; else 16, 32 or 64 */
+  int32_t x_chunk;      /* 0 = engine default; planes per block along x */
+} skx_desc;
+
+typedef struct skx_engine* skx_handle;
+
+/* Result of a device-resident run. */
+typedef struct skx_run_stats {
+  int32_t steps;        /* ANF steps taken */
+  int32_t converged;    /* 1 if the last stage-4 max|g| < tol */
+  double energy;        /* energy of the final field (what the last step() would return) */
+  double err_signed;    /* reference `err`: signed max of the last stage-4 gradient */
+  double err_abs;       /* max |g| of the last stage-4 gradient */
+  double baryon;        /* baryon number of the final field */
+  double seconds;       /* wall time of the run on the host clock */
+} skx_run_stats;
+
+/* One record per step of a run (when a history buffer is attached). */
+typedef struct skx_step_record {
+  double energy;        /* energy of the field entering the step */
+  double err_signed;
+  double err_abs;
+  double baryon;        /* baryon number of the field entering the step */
+} skx_step_record;
+
+/* Replaces Simulation.__init__ buffer allocation + set_params
+ * (core/simulation.py:46-122, :175-197). */
+int skx_create(const skx_desc* desc, skx_handle* out);
+int skx_destroy(skx_handle h);
+
+/* Bytes of device memory skx_create would allocate for this description
+ * (used by admission control; no device work). */
+int skx_footprint_bytes(const skx_desc* desc, size_t* bytes);
+
+/* Replaces cuda.to_device / DeviceNDArray.copy_to_device of Simulation.Field /
+ * .Velocity (core/simulation.py:94-95) and .copy_to_host (:335). n = nf*N. */
+int skx_upload(skx_handle h, int32_t which, const double* host, size_t n);
+int skx_download(skx_handle h, int32_t which, double* host, size_t n);
+
+/* Replaces Simulation.step -> do_arrested_newton_flow
+ * (core/simulation.py:287-308, core/integrator.py:284-410): one RK4 ANF step
+ * with the reference's exact semantics; returns the new energy, the
+ * reference's signed-max `err`, and the true max|g|.  Synchronous. */
+int skx_anf_step(skx_handle h, double e_prev, double* e_new, double* err_signed, double* err_abs);
+
+/* Device-resident loop of skx_anf_step calls (the loop the reference's
+ * viewer drives, theories/nuclear_skyrme/render_gl.py:944-973, with a
+ * convergence test): stops after max_steps or when the stage-4 max|g| < tol.
+ * The arrest decision of each step is taken on the device in FP64.
+ * check_every = steps between host polls (0 = engine default). */
+int skx_anf_run(skx_handle h, int32_t max_steps, double tol, int32_t check_every, skx_run_stats* stats);
+
+/* Plain RK4 steps (killkinen forced off), n steps. */
+int skx_rk4_steps(skx_handle h, int32_t n, skx_run_stats* stats);
+
+/* Replaces observables.compute_energy / compute_skyrmion_number
+ * (theories/nuclear_skyrme/observables.py:114-180). */
+int skx_energy(skx_handle h, double* energy);
+int skx_baryon(skx_handle h, double* baryon);
+
+/* Replaces Simulation.compute_*_density (core/simulation.py:219-285): writes
+ * N doubles (density times cell volume for ENERGY / BARYON). */
+int skx_density(skx_handle h, int32_t kind, double* host, size_t n);
+
+/* Parity probe: evaluate one gradient stage (gradient_step_kernel,
+ * core/integrator.py:37-46) on the given stage field `temp` without touching
+ * the solver state.  eg = projected EnergyGradient, l = -dt*eg; nf*N each
+ * (either may be NULL). */
+int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n);
+
+/* Attach a device history of up to `capacity` skx_step_record entries for the
+ * next runs (0 detaches); skx_history_read copies min(n, steps) records. */
+int skx_history_attach(skx_handle h, int32_t capacity);
+int skx_history_read(skx_handle h, skx_step_record* out, int32_t n, int32_t* n_read);
+
+/* Stage-kernel timing (SKX_FLAG_TRACE): per stage kind 0..4 (0 = energy
+ * pass, 1..4 = RK4 stages), accumulated milliseconds and launch counts. */
+int skx_trace_read(skx_handle h, double ms[5], int64_t counts[5]);
+int skx_trace_reset(skx_handle h);
+
+/* Wait for all work queued on the handle's stream. */
+int skx_sync(skx_handle h);
+
+const char* skx_last_error(void);
+int skx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKX_H */

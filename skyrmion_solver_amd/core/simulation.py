@@ -1,0 +1,196 @@
+"""``Simulation``: the reference's driver API on top of the gfx950 engine.
+
+Mirrors ``core/simulation.py`` of the reference (``Simulation.__init__``
+``:46-122``, ``initialize`` ``:124-173``, ``set_params`` ``:175-197``,
+``observables`` ``:199-217``, the density helpers ``:219-285`` and
+``step`` ``:287-308``) with the same method names, arguments and return
+values.  The per-step work runs in the engine's HIP kernels through the C ABI
+(``include/skx.h``); there is no CPU path.
+
+Device buffers the reference exposes as Numba ``DeviceNDArray`` objects
+(``Field``, ``Velocity``, ``en``) are exposed as :class:`DeviceBuffer`, which
+keeps the ``copy_to_host`` / ``copy_to_device`` methods the reference's
+callers use.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from skyrmion_solver_amd import _engine
+from skyrmion_solver_amd import initial_data
+from skyrmion_solver_amd.core.params import Params
+
+PRECISIONS = {"fp64": _engine.FP64, "float64": _engine.FP64, "fp32": _engine.FP32, "float32": _engine.FP32}
+
+
+class DeviceBuffer:
+    """A field-sized engine buffer with the DeviceNDArray copy methods."""
+
+    def __init__(self, sim: "Simulation", which: int):
+        self._sim = sim
+        self._which = which
+
+    @property
+    def size(self) -> int:
+        return int(self._sim.rp.dim_fields)
+
+    @property
+    def shape(self):
+        return (self.size,)
+
+    dtype = np.dtype(np.float64)
+
+    def copy_to_host(self) -> np.ndarray:
+        return self._sim.engine.download(self._which)
+
+    def copy_to_device(self, host) -> None:
+        self._sim.engine.upload(self._which, np.asarray(host, dtype=np.float64))
+
+
+class HostDensity:
+    """The last density computed by a ``compute_*_density`` call (the reference's ``en``)."""
+
+    def __init__(self, values: np.ndarray | None = None):
+        self.values = values
+
+    def copy_to_host(self) -> np.ndarray:
+        if self.values is None:
+            raise RuntimeError("no density has been computed yet")
+        return self.values.copy()
+
+
+class Simulation:
+    """Bind a theory to the engine (``core/simulation.py:28-122``).
+
+    Parameters
+    ----------
+    params : Params
+        Theory parameter set (resolved with ``params.resolved()``).
+    theory : module
+        A theory package from :func:`skyrmion_solver_amd.theories.load_theory`.
+    precision : str
+        ``"fp64"`` (plain form, reference arithmetic) or ``"fp32"`` (FP32
+        storage and stencils, FP64 reductions and renormalisation).
+    device : int
+        HIP device ordinal.
+    engine_options : dict
+        ``flags``, ``tile_z``, ``x_chunk`` passed to ``skx_create``.
+    """
+
+    def __init__(self, params, theory, precision: str = "fp64", device: int = 0, engine_options: dict | None = None):
+        self.theory = theory
+        self.kernels = theory.kernels
+        self.initial_config = theory.initial_config
+        self.observables_mod = theory.observables
+        self.threads3d = getattr(theory, "threads3d", (8, 8, 4))
+        if getattr(self.kernels, "THEORY_ID", None) is None:
+            raise AttributeError("Theory kernels must provide THEORY_ID (the engine theory this plugin runs on).")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+        self.precision = precision
+        self.device = device
+        self.engine_options = dict(engine_options or {})
+        self.engine = None
+        self.set_params(params)
+        self.en = HostDensity()
+        self.entmp = HostDensity()
+
+    # ------------------------------------------------------------ parameters
+    def set_params(self, p):
+        """Update the parameters (``core/simulation.py:175-197``).
+
+        The packed arrays are handed to a fresh engine handle; an existing
+        field and velocity are carried over when the lattice size is unchanged.
+        """
+        old = self.engine
+        carry = None
+        self.p = p
+        self.rp = p if not isinstance(p, Params) else p.resolved()
+        self.p_i_h, self.p_f_h = self.theory.params.pack_device_params(self.rp)
+        if old is not None and int(old.p_i[7]) == int(self.p_i_h[7]):
+            carry = (old.download(_engine.FIELD), old.download(_engine.VELOCITY))
+        if old is not None:
+            old.close()
+        self.engine = _engine.Engine(self.kernels.THEORY_ID, PRECISIONS[self.precision], self.p_i_h, self.p_f_h,
+                                     device=self.device, **self.engine_options)
+        self.Field = DeviceBuffer(self, _engine.FIELD)
+        self.Velocity = DeviceBuffer(self, _engine.VELOCITY)
+        if carry is not None:
+            self.engine.upload(_engine.FIELD, carry[0])
+            self.engine.upload(_engine.VELOCITY, carry[1])
+
+    # ------------------------------------------------------------ initial data
+    def initialize(self, init_config: dict | None = None):
+        """Set the initial field and zero the velocity (``core/simulation.py:124-173``)."""
+        cfg = dict(init_config or {})
+        F = self.initial_config.initialize(sim=self, config=cfg)
+        self.engine.upload(_engine.FIELD, F)
+        self.engine.upload(_engine.VELOCITY, np.zeros_like(F))
+        post = getattr(self.initial_config, "post_initialize", None)
+        if post is not None:
+            post(sim=self, config=cfg)
+
+    # ------------------------------------------------------------ observables
+    def observables(self) -> dict:
+        """Energy and baryon number (``core/simulation.py:199-217``)."""
+        obs = {"energy": self.observables_mod.compute_energy(self)}
+        if hasattr(self.observables_mod, "compute_skyrmion_number"):
+            obs["skyrmion_number"] = self.observables_mod.compute_skyrmion_number(self)
+        return obs
+
+    def compute_energy_density(self):
+        """Per-site energy (times cell volume) into ``en`` (``core/simulation.py:219-234``)."""
+        self.en = HostDensity(self.engine.density(_engine.DENSITY_ENERGY))
+
+    def compute_baryon_density(self):
+        """Per-site baryon density (times cell volume) into ``en`` (``:236-251``)."""
+        self.en = HostDensity(self.engine.density(_engine.DENSITY_BARYON))
+
+    def compute_potential_density(self):
+        """The omega / Coulomb potential component into ``en`` (``:253-268``)."""
+        self.en = HostDensity(self.engine.density(_engine.DENSITY_POTENTIAL))
+
+    def compute_rho_meson_density(self):
+        """Sum of squared rho components into ``en`` (``:270-285``)."""
+        self.en = HostDensity(self.engine.density(_engine.DENSITY_RHO))
+
+    # ------------------------------------------------------------ the hot path
+    def step(self, prev_energy: float) -> tuple[float, float]:
+        """One arrested-Newton-flow RK4 step (``core/simulation.py:287-308``).
+
+        Returns ``(new_energy, err)`` with the reference's ``err`` (the signed
+        maximum of the stage-4 energy gradient).  The true ``max |g|`` of the
+        same gradient is kept in ``self.last_max_force``.
+        """
+        e, err, err_abs = self.engine.anf_step(prev_energy)
+        self.last_max_force = err_abs
+        return e, err
+
+    def minimize(self, tol: float = 1.0e-4, max_steps: int = 100000, check_every: int = 0) -> dict:
+        """Run ANF on the device until the stage-4 ``max |g| < tol``.
+
+        The reference documents ``minimize(tol=...)`` (``core/simulation.py:8,43``)
+        but has no such method; this is the device-resident loop of ``step``.
+        """
+        return self.engine.anf_run(max_steps, tol, check_every).as_dict()
+
+    def save_output(self, output_dir: str | None = None, precision: int = 32):
+        """Checkpoint the field and velocity as ``.npz`` (the reference's text bundle writer is not rebuilt)."""
+        from pathlib import Path
+
+        out = Path(output_dir or ".")
+        out.mkdir(parents=True, exist_ok=True)
+        np.savez(out / "state.npz", Field=self.engine.download(_engine.FIELD),
+                 Velocity=self.engine.download(_engine.VELOCITY), p_i=self.p_i_h, p_f=self.p_f_h)
+
+    def print_instructions(self):
+        if hasattr(self.theory, "print_instructions"):
+            self.theory.print_instructions()
+        else:
+            print("No specific instructions available for this theory.")
+
+    def close(self):
+        if self.engine is not None:
+            self.engine.close()
+            self.engine = None

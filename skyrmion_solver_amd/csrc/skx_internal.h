@@ -1,0 +1,92 @@
+// Internal declarations shared by the kernel and runtime translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "skx_physics.h"
+
+namespace skx {
+
+// Device-resident solver status.  Written only by the last block of a
+// reducing kernel (thread 0, ordinary vector stores) or by the host between
+// runs; read by the kernels of the next launch.
+struct DevStatus {
+  double energy;       // last energy reduced (sum of e*vol)
+  double baryon;       // last baryon number reduced
+  double e_prev;       // energy the next arrest decision compares against
+  double err_signed;   // signed max of the last stage-4 gradient (reference "err")
+  double err_abs;      // max |g| of the last stage-4 gradient
+  double keep;         // 1.0: use V as stored; 0.0: V is arrested (treated as 0)
+  double tol;          // convergence threshold on err_abs
+  int32_t arrest_pending;  // V was produced by a step whose arrest is not decided yet
+  int32_t done;            // run finished (converged, step limit, or non-finite)
+  int32_t steps;           // steps completed in the current run
+  int32_t max_steps;       // step limit of the current run
+  int32_t nonfinite;       // a non-finite energy or gradient was seen
+  int32_t killkinen;       // arrest enabled (reference p_i[8])
+  uint32_t cnt_sum;        // last-block counters (reset by the last block)
+  uint32_t cnt_max;
+  int32_t hist_cap;        // capacity of the optional per-step history
+  int32_t pad_;
+};
+
+// Per-step record kept on the device when a history buffer is attached.
+struct StepRecord {
+  double energy;      // energy of the field entering the step
+  double err_signed;  // stage-4 signed max
+  double err_abs;     // stage-4 max |g|
+  double baryon;      // baryon number of the field entering the step
+};
+
+enum StageFlags : int {
+  FUSE_ENERGY = 1,   // stage 1 / 0: reduce the energy of the stage field
+  FUSE_BARYON = 2,   // stage 1 / 0: reduce the baryon number as well
+  CHECK_DONE = 4,    // exit early when st->done is set
+  DECIDE_ARREST = 8, // stage 1 / 0: make the pending arrest decision from the reduced energy
+  RUN_MODE = 16,     // stage 4: count steps / test convergence in st
+};
+
+template <typename T>
+struct StageArgs {
+  const T* F;    // field (stage field base)
+  const T* V;    // velocity
+  const T* la;   // stage 3: l1, stage 4: l2
+  const T* lb;   // stage 4: l1
+  const T* lc;   // stage 4: l3
+  T* lout;       // stages 1-3: l_s
+  T* Fout;       // stage 4: new field
+  T* Vout;       // stage 4: new velocity
+  double* eg_out;   // optional: projected gradient (double) of this stage
+  double* dens_e;   // optional (stage 0): energy density per site
+  double* dens_b;   // optional (stage 0): baryon density per site
+  DevStatus* st;
+  double* partial;  // per-block reduction partials, 2 per block
+  StepRecord* hist; // optional per-step history
+  int X, Y, Z, halo;
+  int xchunk;
+  int nbz, nby, nbx;
+  long long N;      // sites per component
+  T ilsx, ilsy, ilsz;
+  PhysParams P;
+  int unit_length;
+  int flags;
+};
+
+// Kernel launchers (skx_kernels.hip).  Return hipSuccess or the launch error.
+struct LaunchShape {
+  int tz, ty, xchunk;
+};
+
+hipError_t launch_stage_f64(int theory, int stage, const StageArgs<double>& a, const LaunchShape& s, hipStream_t st);
+hipError_t launch_stage_f32(int theory, int stage, const StageArgs<float>& a, const LaunchShape& s, hipStream_t st);
+int stage_blocks(int X, int Y, int Z, const LaunchShape& s);
+bool shape_supported(const LaunchShape& s);
+
+hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);  // zero fill kernel (graph safe)
+hipError_t launch_apply_arrest_f64(double* V, long long n, const DevStatus* st, hipStream_t s);
+hipError_t launch_apply_arrest_f32(float* V, long long n, const DevStatus* st, hipStream_t s);
+
+}  // namespace skx

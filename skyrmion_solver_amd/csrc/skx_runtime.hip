@@ -1,0 +1,713 @@
+// Host runtime and C ABI of the engine (include/skx.h).
+//
+// A handle owns the device state of one lattice:
+//   F[2], V[2]   field / velocity, ping-ponged by the stage-4 kernel
+//   l[3]         RK4 velocity slopes l1, l2, l3 (l4 never leaves registers)
+//   st           DevStatus (energies, arrest flag, convergence, counters)
+//   partial      per-block reduction partials
+// and one HIP stream.  Stage kernels are launched directly or replayed from
+// a captured hipGraph of a chunk of steps.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/skx.h"
+#include "skx_internal.h"
+
+using namespace skx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                                    \
+  do {                                                                                                   \
+    hipError_t _e = (expr);                                                                              \
+    if (_e != hipSuccess) return fail(SKX_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+  } while (0)
+
+struct Lattice {
+  int theory = 0, prec = 0, nf = 0, nsk = 4;
+  int X = 0, Y = 0, Z = 0, halo = 2;
+  long long N = 0;
+  double lsx = 0, lsy = 0, lsz = 0;
+  PhysParams P{};
+  int killkinen = 1, unit_length = 1;
+};
+
+int parse_desc(const skx_desc* d, Lattice& L) {
+  if (!d || !d->p_i || !d->p_f) return fail(SKX_EINVAL, "null descriptor or parameter array");
+  if (d->n_i < 11 || d->n_f < 8) return fail(SKX_EINVAL, "packed parameter arrays too short (need p_i[11], p_f[8])");
+  if (d->precision != SKX_FP64 && d->precision != SKX_FP32) return fail(SKX_EINVAL, "unknown precision");
+  const int32_t* pi = d->p_i;
+  const double* pf = d->p_f;
+  L.theory = d->theory;
+  L.prec = d->precision;
+  L.X = pi[0];
+  L.Y = pi[1];
+  L.Z = pi[2];
+  L.halo = pi[3];
+  L.nf = pi[5];
+  if (L.X < 1 || L.Y < 1 || L.Z < 1) return fail(SKX_EINVAL, "lattice dimensions must be positive");
+  if (L.halo < 2) return fail(SKX_EINVAL, "halo (p_i[3]) must be >= 2 for the radius-2 stencils");
+  if (pi[4] != 3) return fail(SKX_EINVAL, "number_coordinates (p_i[4]) must be 3");
+  L.N = (long long)L.X * L.Y * L.Z;
+  if (pi[6] != L.N) return fail(SKX_EINVAL, "p_i[6] (dim_grid) != X*Y*Z");
+  if ((long long)pi[7] != L.N * L.nf) return fail(SKX_EINVAL, "p_i[7] (dim_fields) != nf*N");
+  if (L.N * L.nf >= (1LL << 31)) return fail(SKX_EINVAL, "lattice too large for one handle (nf*N >= 2^31)");
+  L.killkinen = pi[8] ? 1 : 0;
+  L.unit_length = pi[10] ? 1 : 0;
+  L.lsx = pf[3];
+  L.lsy = pf[4];
+  L.lsz = pf[5];
+  if (!(L.lsx > 0) || !(L.lsy > 0) || !(L.lsz > 0)) return fail(SKX_EINVAL, "lattice spacings must be positive");
+  L.P.vol = pf[6];
+  L.P.dt = pf[7];
+  switch (L.theory) {
+    case SKX_THEORY_PION:
+      if (L.nf != 4) return fail(SKX_EINVAL, "pion theory needs number_total_fields == 4");
+      if (d->n_f < 10) return fail(SKX_EINVAL, "pion theory needs p_f[8] (mpi) and p_f[9] (kappa)");
+      L.P.m2 = pf[8] * pf[8];
+      L.P.k2 = pf[9] * pf[9];
+      break;
+    case SKX_THEORY_OMEGA:
+    case SKX_THEORY_COULOMB:
+      if (L.nf != 5) return fail(SKX_EINVAL, "omega / Coulomb theories need number_total_fields == 5");
+      if (d->n_i < 12 || pi[11] != 4) return fail(SKX_EINVAL, "omega / Coulomb theories need p_i[11] == 4");
+      if (d->n_f < 10) return fail(SKX_EINVAL, "omega / Coulomb theories need p_f[8] and p_f[9]");
+      L.P.m2 = pf[8] * pf[8];
+      L.P.coup = pf[9];
+      break;
+    case SKX_THEORY_RHO:
+      if (L.nf != 13) return fail(SKX_EINVAL, "rho theory needs number_total_fields == 13");
+      if (d->n_f < 11) return fail(SKX_EINVAL, "rho theory needs p_f[8..10]");
+      L.P.m2 = pf[8] * pf[8];
+      L.P.coup = pf[9] * pf[9];
+      L.P.calpha = pf[10];
+      return fail(SKX_ENOTSUP, "rho theory kernels are not built in this version");
+    default:
+      return fail(SKX_EINVAL, "unknown theory id");
+  }
+  return SKX_OK;
+}
+
+LaunchShape choose_shape(const Lattice& L, const skx_desc* d) {
+  LaunchShape s{32, 8, 16};
+  if (d->tile_z == 64) s = {64, 4, 16};
+  if (d->tile_z == 16) s = {16, 16, 16};
+  if (d->tile_z == 0 && L.Z <= 16) s = {16, 16, 16};
+  if (d->x_chunk > 0) s.xchunk = d->x_chunk;
+  else {
+    // enough blocks to fill 256 CUs several times over, but chunks no shorter than 8 planes
+    const int cols = ((L.Z + s.tz - 1) / s.tz) * ((L.Y + s.ty - 1) / s.ty);
+    int xc = L.X;
+    while (xc > 8 && (long long)cols * ((L.X + xc - 1) / xc) < 2048) xc = (xc + 1) / 2;
+    s.xchunk = xc;
+  }
+  return s;
+}
+
+size_t elem_size(int prec) { return prec == SKX_FP64 ? sizeof(double) : sizeof(float); }
+
+}  // namespace
+
+struct skx_engine {
+  Lattice L;
+  int device = 0;
+  uint32_t flags = 0;
+  LaunchShape shape{};
+  hipStream_t stream = nullptr;
+  void* F[2] = {nullptr, nullptr};
+  void* V[2] = {nullptr, nullptr};
+  void* l[3] = {nullptr, nullptr, nullptr};
+  int cur = 0;
+  DevStatus* st = nullptr;
+  DevStatus* st_host = nullptr;  // pinned mirror
+  double* partial = nullptr;
+  double* eg = nullptr;     // lazily allocated probe / density buffer (double, nf*N)
+  double* dens2 = nullptr;  // lazily allocated second density buffer (double, N)
+  StepRecord* hist = nullptr;
+  int hist_cap = 0;
+  // captured chunks of `graph_steps` steps, one per starting parity
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_steps = 0;
+  int graph_flags = 0;
+  // tracing
+  double trace_ms[5] = {0, 0, 0, 0, 0};
+  long long trace_n[5] = {0, 0, 0, 0, 0};
+  struct TracedLaunch {
+    int stage;
+    hipEvent_t e0, e1;
+  };
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<TracedLaunch> ev_pending;
+};
+
+namespace {
+
+size_t footprint(const Lattice& L, const LaunchShape& s) {
+  const size_t arr = (size_t)L.N * L.nf * elem_size(L.prec);
+  const size_t nblk = (size_t)stage_blocks(L.X, L.Y, L.Z, s);
+  return 7 * arr + 2 * nblk * sizeof(double) + sizeof(DevStatus);
+}
+
+template <typename T>
+StageArgs<T> make_args(skx_engine* e, int stage, int flags, int c, const void* field_override = nullptr) {
+  const Lattice& L = e->L;
+  StageArgs<T> a{};
+  a.F = static_cast<const T*>(field_override ? field_override : e->F[c]);
+  a.V = static_cast<const T*>(e->V[c]);
+  a.la = nullptr;
+  a.lb = nullptr;
+  a.lc = nullptr;
+  a.lout = nullptr;
+  a.Fout = nullptr;
+  a.Vout = nullptr;
+  if (stage == 1) a.lout = static_cast<T*>(e->l[0]);
+  if (stage == 2) a.lout = static_cast<T*>(e->l[1]);
+  if (stage == 3) {
+    a.la = static_cast<const T*>(e->l[0]);
+    a.lout = static_cast<T*>(e->l[2]);
+  }
+  if (stage == 4) {
+    a.la = static_cast<const T*>(e->l[1]);
+    a.lb = static_cast<const T*>(e->l[0]);
+    a.lc = static_cast<const T*>(e->l[2]);
+    a.Fout = static_cast<T*>(e->F[c ^ 1]);
+    a.Vout = static_cast<T*>(e->V[c ^ 1]);
+  }
+  a.eg_out = nullptr;
+  a.dens_e = nullptr;
+  a.dens_b = nullptr;
+  a.st = e->st;
+  a.partial = e->partial;
+  a.hist = e->hist;
+  a.X = L.X;
+  a.Y = L.Y;
+  a.Z = L.Z;
+  a.halo = L.halo;
+  a.xchunk = e->shape.xchunk;
+  a.N = L.N;
+  a.ilsx = T(1.0 / L.lsx);
+  a.ilsy = T(1.0 / L.lsy);
+  a.ilsz = T(1.0 / L.lsz);
+  a.P = L.P;
+  a.unit_length = L.unit_length;
+  a.flags = flags;
+  return a;
+}
+
+hipEvent_t pooled_event(skx_engine* e) {
+  if (!e->ev_pool.empty()) {
+    hipEvent_t ev = e->ev_pool.back();
+    e->ev_pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev = nullptr;
+  (void)hipEventCreate(&ev);
+  return ev;
+}
+
+// Fold the timings of completed traced launches into the per-stage totals.
+void resolve_trace(skx_engine* e) {
+  for (auto& t : e->ev_pending) {
+    (void)hipEventSynchronize(t.e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t.e0, t.e1);
+    e->trace_ms[t.stage] += ms;
+    e->trace_n[t.stage] += 1;
+    e->ev_pool.push_back(t.e0);
+    e->ev_pool.push_back(t.e1);
+  }
+  e->ev_pending.clear();
+}
+
+hipError_t launch(skx_engine* e, int stage, int flags, int c, double* eg_out = nullptr, double* dens_e = nullptr,
+                  double* dens_b = nullptr, const void* field_override = nullptr) {
+  const bool trace = (e->flags & SKX_FLAG_TRACE) != 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (trace) {
+    e0 = pooled_event(e);
+    e1 = pooled_event(e);
+    (void)hipEventRecord(e0, e->stream);
+  }
+  hipError_t r;
+  if (e->L.prec == SKX_FP64) {
+    StageArgs<double> a = make_args<double>(e, stage, flags, c, field_override);
+    a.eg_out = eg_out;
+    a.dens_e = dens_e;
+    a.dens_b = dens_b;
+    r = launch_stage_f64(e->L.theory, stage, a, e->shape, e->stream);
+  } else {
+    StageArgs<float> a = make_args<float>(e, stage, flags, c, field_override);
+    a.eg_out = eg_out;
+    a.dens_e = dens_e;
+    a.dens_b = dens_b;
+    r = launch_stage_f32(e->L.theory, stage, a, e->shape, e->stream);
+  }
+  if (trace) {
+    (void)hipEventRecord(e1, e->stream);
+    e->ev_pending.push_back({stage, e0, e1});
+  }
+  return r;
+}
+
+int push_status(skx_engine* e) {
+  HIP_TRY(hipMemcpyAsync(e->st, e->st_host, sizeof(DevStatus), hipMemcpyHostToDevice, e->stream));
+  return SKX_OK;
+}
+int pull_status(skx_engine* e) {
+  HIP_TRY(hipMemcpyAsync(e->st_host, e->st, sizeof(DevStatus), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  resolve_trace(e);
+  return SKX_OK;
+}
+
+// One ANF step of the device-resident loop from parity c (no host sync).
+hipError_t enqueue_run_step(skx_engine* e, int c, int fuse_baryon) {
+  const int s1 = FUSE_ENERGY | DECIDE_ARREST | CHECK_DONE | (fuse_baryon ? FUSE_BARYON : 0);
+  hipError_t r;
+  if ((r = launch(e, 1, s1, c)) != hipSuccess) return r;
+  if ((r = launch(e, 2, CHECK_DONE, c)) != hipSuccess) return r;
+  if ((r = launch(e, 3, CHECK_DONE, c)) != hipSuccess) return r;
+  return launch(e, 4, RUN_MODE | CHECK_DONE, c);
+}
+
+void drop_graphs(skx_engine* e) {
+  for (int i = 0; i < 2; ++i)
+    if (e->graph[i]) {
+      (void)hipGraphExecDestroy(e->graph[i]);
+      e->graph[i] = nullptr;
+    }
+  e->graph_steps = 0;
+}
+
+int build_graph(skx_engine* e, int parity, int steps, int fuse_baryon) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t r = hipSuccess;
+  for (int s = 0; s < steps && r == hipSuccess; ++s) r = enqueue_run_step(e, (parity + s) & 1, fuse_baryon);
+  hipError_t r2 = hipStreamEndCapture(e->stream, &g);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("stage launch during capture: ") + hipGetErrorString(r));
+  if (r2 != hipSuccess) return fail(SKX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(r2));
+  hipGraphExec_t ex = nullptr;
+  hipError_t r3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (r3 != hipSuccess) return fail(SKX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r3));
+  e->graph[parity] = ex;
+  return SKX_OK;
+}
+
+int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, int killkinen, skx_run_stats* out) {
+  auto t0 = std::chrono::steady_clock::now();
+  DevStatus* h = e->st_host;
+  std::memset(h, 0, sizeof(DevStatus));
+  h->keep = 1.0;
+  h->tol = tol;
+  h->max_steps = max_steps;
+  h->killkinen = killkinen;
+  h->hist_cap = e->hist ? e->hist_cap : 0;
+  if (int rc = push_status(e)) return rc;
+  const int c0 = e->cur;
+  int chunk = check_every > 0 ? check_every : 32;
+  const bool use_graph = !(e->flags & (SKX_FLAG_NO_GRAPH | SKX_FLAG_TRACE));
+  const int fuse_baryon = e->hist ? 1 : 0;
+  if (use_graph && (e->graph_steps != chunk || e->graph_flags != fuse_baryon)) {
+    drop_graphs(e);
+    e->graph_steps = chunk;
+    e->graph_flags = fuse_baryon;
+  }
+  int issued = 0;
+  while (issued < max_steps) {
+    // the launches of a chunk start from the parity the device will be at
+    // if no step is skipped; skipped steps (done flag) stop the run anyway
+    const int parity = (c0 + issued) & 1;
+    if (use_graph) {
+      if (!e->graph[parity]) {
+        if (int rc = build_graph(e, parity, chunk, fuse_baryon)) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(e->graph[parity], e->stream));
+      issued += chunk;
+    } else {
+      const int n = std::min(chunk, max_steps - issued);
+      for (int s = 0; s < n; ++s) {
+        hipError_t r = enqueue_run_step(e, (parity + s) & 1, fuse_baryon);
+        if (r != hipSuccess) return fail(SKX_EHIP, std::string("stage launch: ") + hipGetErrorString(r));
+      }
+      issued += n;
+    }
+    if (int rc = pull_status(e)) return rc;
+    if (h->done) break;
+  }
+  const int steps = h->steps;
+  e->cur = (c0 + steps) & 1;
+  // energy of the final field, the last step's arrest decision, and its application
+  {
+    hipError_t r = launch(e, 0, FUSE_ENERGY | FUSE_BARYON | DECIDE_ARREST, e->cur);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("energy pass: ") + hipGetErrorString(r));
+    const long long n = e->L.N * e->L.nf;
+    r = e->L.prec == SKX_FP64 ? launch_apply_arrest_f64(static_cast<double*>(e->V[e->cur]), n, e->st, e->stream)
+                              : launch_apply_arrest_f32(static_cast<float*>(e->V[e->cur]), n, e->st, e->stream);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("arrest: ") + hipGetErrorString(r));
+  }
+  if (int rc = pull_status(e)) return rc;
+  auto t1 = std::chrono::steady_clock::now();
+  if (out) {
+    out->steps = steps;
+    out->converged = (steps > 0 && h->err_abs < tol) ? 1 : 0;
+    out->energy = h->energy;
+    out->err_signed = h->err_signed;
+    out->err_abs = h->err_abs;
+    out->baryon = h->baryon;
+    out->seconds = std::chrono::duration<double>(t1 - t0).count();
+  }
+  if (h->nonfinite) return fail(SKX_ENONFINITE, "non-finite energy or gradient during the run");
+  return SKX_OK;
+}
+
+int ensure_eg(skx_engine* e) {
+  if (e->eg) return SKX_OK;
+  if (hipMalloc(&e->eg, (size_t)e->L.N * e->L.nf * sizeof(double)) != hipSuccess) {
+    e->eg = nullptr;
+    return fail(SKX_ENOMEM, "probe buffer allocation failed");
+  }
+  return SKX_OK;
+}
+int ensure_dens2(skx_engine* e) {
+  if (e->dens2) return SKX_OK;
+  if (hipMalloc(&e->dens2, (size_t)e->L.N * sizeof(double)) != hipSuccess) {
+    e->dens2 = nullptr;
+    return fail(SKX_ENOMEM, "density buffer allocation failed");
+  }
+  return SKX_OK;
+}
+
+int copy_in(skx_engine* e, void* dst, const double* host, size_t n) {
+  if (e->L.prec == SKX_FP64) {
+    HIP_TRY(hipMemcpy(dst, host, n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = static_cast<float>(host[i]);
+    HIP_TRY(hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return SKX_OK;
+}
+int copy_out(skx_engine* e, double* host, const void* src, size_t n) {
+  if (e->L.prec == SKX_FP64) {
+    HIP_TRY(hipMemcpy(host, src, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) host[i] = static_cast<double>(tmp[i]);
+  }
+  return SKX_OK;
+}
+
+void release(skx_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graphs(e);
+  resolve_trace(e);
+  for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+  for (void* p : {e->F[0], e->F[1], e->V[0], e->V[1], e->l[0], e->l[1], e->l[2]})
+    if (p) (void)hipFree(p);
+  if (e->st) (void)hipFree(e->st);
+  if (e->st_host) (void)hipHostFree(e->st_host);
+  if (e->partial) (void)hipFree(e->partial);
+  if (e->eg) (void)hipFree(e->eg);
+  if (e->dens2) (void)hipFree(e->dens2);
+  if (e->hist) (void)hipFree(e->hist);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+#define CHECK_HANDLE(h)                                                        \
+  do {                                                                         \
+    if (!(h)) return fail(SKX_EINVAL, "null handle");                          \
+    if (hipSetDevice((h)->device) != hipSuccess) return fail(SKX_EHIP, "hipSetDevice failed"); \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int skx_version(void) { return SKX_VERSION; }
+
+const char* skx_last_error(void) { return g_last_error.c_str(); }
+
+int skx_footprint_bytes(const skx_desc* desc, size_t* bytes) {
+  Lattice L;
+  if (int rc = parse_desc(desc, L)) return rc;
+  if (!bytes) return fail(SKX_EINVAL, "null output");
+  *bytes = footprint(L, choose_shape(L, desc));
+  return SKX_OK;
+}
+
+int skx_create(const skx_desc* desc, skx_handle* out) {
+  if (!out) return fail(SKX_EINVAL, "null output handle");
+  *out = nullptr;
+  Lattice L;
+  if (int rc = parse_desc(desc, L)) return rc;
+  if (desc->tile_z != 0 && desc->tile_z != 16 && desc->tile_z != 32 && desc->tile_z != 64)
+    return fail(SKX_EINVAL, "tile_z must be 0, 16, 32 or 64");
+  skx_engine* e = new skx_engine();
+  e->L = L;
+  e->device = desc->device;
+  e->flags = desc->flags;
+  e->shape = choose_shape(L, desc);
+  if (!shape_supported(e->shape)) {
+    delete e;
+    return fail(SKX_EINVAL, "unsupported launch shape");
+  }
+  hipError_t r = hipSetDevice(e->device);
+  if (r != hipSuccess) {
+    delete e;
+    return fail(SKX_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(r));
+  }
+  if ((r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
+    e->stream = nullptr;
+    release(e);
+    return fail(SKX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(r));
+  }
+  const size_t arr = (size_t)L.N * L.nf * elem_size(L.prec);
+  void** bufs[7] = {&e->F[0], &e->F[1], &e->V[0], &e->V[1], &e->l[0], &e->l[1], &e->l[2]};
+  for (void** b : bufs) {
+    if (hipMalloc(b, arr) != hipSuccess) {
+      *b = nullptr;
+      release(e);
+      return fail(SKX_ENOMEM, "device allocation of lattice arrays failed");
+    }
+  }
+  const size_t nblk = (size_t)stage_blocks(L.X, L.Y, L.Z, e->shape);
+  if (hipMalloc(&e->partial, 2 * nblk * sizeof(double)) != hipSuccess ||
+      hipMalloc(&e->st, sizeof(DevStatus)) != hipSuccess ||
+      hipHostMalloc(&e->st_host, sizeof(DevStatus), hipHostMallocDefault) != hipSuccess) {
+    release(e);
+    return fail(SKX_ENOMEM, "device allocation of reduction state failed");
+  }
+  // zero everything with a fill kernel (no memset nodes anywhere)
+  for (void** b : bufs) (void)launch_fill(*b, arr, e->stream);
+  (void)launch_fill(e->partial, 2 * nblk * sizeof(double), e->stream);
+  std::memset(e->st_host, 0, sizeof(DevStatus));
+  e->st_host->keep = 1.0;
+  if (int rc = push_status(e)) {
+    release(e);
+    return rc;
+  }
+  if ((r = hipStreamSynchronize(e->stream)) != hipSuccess) {
+    release(e);
+    return fail(SKX_EHIP, std::string("initialisation: ") + hipGetErrorString(r));
+  }
+  *out = e;
+  return SKX_OK;
+}
+
+int skx_destroy(skx_handle h) {
+  if (!h) return SKX_OK;
+  release(h);
+  return SKX_OK;
+}
+
+int skx_upload(skx_handle h, int32_t which, const double* host, size_t n) {
+  CHECK_HANDLE(h);
+  if (!host) return fail(SKX_EINVAL, "null host array");
+  if (n != (size_t)(h->L.N * h->L.nf)) return fail(SKX_EINVAL, "upload size must be nf*N");
+  if (which != SKX_FIELD && which != SKX_VELOCITY) return fail(SKX_EINVAL, "unknown buffer");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return copy_in(h, which == SKX_FIELD ? h->F[h->cur] : h->V[h->cur], host, n);
+}
+
+int skx_download(skx_handle h, int32_t which, double* host, size_t n) {
+  CHECK_HANDLE(h);
+  if (!host) return fail(SKX_EINVAL, "null host array");
+  if (n != (size_t)(h->L.N * h->L.nf)) return fail(SKX_EINVAL, "download size must be nf*N");
+  if (which != SKX_FIELD && which != SKX_VELOCITY) return fail(SKX_EINVAL, "unknown buffer");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return copy_out(h, host, which == SKX_FIELD ? h->F[h->cur] : h->V[h->cur], n);
+}
+
+int skx_anf_step(skx_handle h, double e_prev, double* e_new, double* err_signed, double* err_abs) {
+  CHECK_HANDLE(h);
+  DevStatus* s = h->st_host;
+  std::memset(s, 0, sizeof(DevStatus));
+  s->keep = 1.0;
+  if (int rc = push_status(h)) return rc;
+  const int c = h->cur;
+  hipError_t r = hipSuccess;
+  for (int stage = 1; stage <= 4 && r == hipSuccess; ++stage) r = launch(h, stage, 0, c);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("stage launch: ") + hipGetErrorString(r));
+  h->cur = c ^ 1;
+  if ((r = launch(h, 0, FUSE_ENERGY, h->cur)) != hipSuccess)
+    return fail(SKX_EHIP, std::string("energy pass: ") + hipGetErrorString(r));
+  if (int rc = pull_status(h)) return rc;
+  const double E = s->energy;
+  // arrest: core/integrator.py:403-407 (strict >, FP64 host energies)
+  if (h->L.killkinen && (E > e_prev)) {
+    const size_t arr = (size_t)h->L.N * h->L.nf * elem_size(h->L.prec);
+    HIP_TRY(launch_fill(h->V[h->cur], arr, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  if (e_new) *e_new = E;
+  if (err_signed) *err_signed = s->err_signed;
+  if (err_abs) *err_abs = s->err_abs;
+  if (s->nonfinite || !std::isfinite(E)) return fail(SKX_ENONFINITE, "non-finite energy or gradient");
+  return SKX_OK;
+}
+
+int skx_anf_run(skx_handle h, int32_t max_steps, double tol, int32_t check_every, skx_run_stats* stats) {
+  CHECK_HANDLE(h);
+  if (max_steps < 0) return fail(SKX_EINVAL, "max_steps must be >= 0");
+  if (check_every < 0) return fail(SKX_EINVAL, "check_every must be >= 0");
+  return run_loop(h, max_steps, tol, check_every, h->L.killkinen, stats);
+}
+
+int skx_rk4_steps(skx_handle h, int32_t n, skx_run_stats* stats) {
+  CHECK_HANDLE(h);
+  if (n < 0) return fail(SKX_EINVAL, "n must be >= 0");
+  return run_loop(h, n, -1.0, 0, 0, stats);
+}
+
+int skx_energy(skx_handle h, double* energy) {
+  CHECK_HANDLE(h);
+  if (!energy) return fail(SKX_EINVAL, "null output");
+  hipError_t r = launch(h, 0, FUSE_ENERGY, h->cur);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("energy pass: ") + hipGetErrorString(r));
+  if (int rc = pull_status(h)) return rc;
+  *energy = h->st_host->energy;
+  return SKX_OK;
+}
+
+int skx_baryon(skx_handle h, double* baryon) {
+  CHECK_HANDLE(h);
+  if (!baryon) return fail(SKX_EINVAL, "null output");
+  hipError_t r = launch(h, 0, FUSE_BARYON, h->cur);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("baryon pass: ") + hipGetErrorString(r));
+  if (int rc = pull_status(h)) return rc;
+  *baryon = h->st_host->baryon;
+  return SKX_OK;
+}
+
+int skx_density(skx_handle h, int32_t kind, double* host, size_t n) {
+  CHECK_HANDLE(h);
+  if (!host) return fail(SKX_EINVAL, "null host array");
+  if (n != (size_t)h->L.N) return fail(SKX_EINVAL, "density size must be N");
+  if (kind == SKX_DENSITY_ENERGY || kind == SKX_DENSITY_BARYON) {
+    if (int rc = ensure_dens2(h)) return rc;
+    const bool en = kind == SKX_DENSITY_ENERGY;
+    hipError_t r = launch(h, 0, en ? FUSE_ENERGY : FUSE_BARYON, h->cur, nullptr, en ? h->dens2 : nullptr,
+                          en ? nullptr : h->dens2);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("density pass: ") + hipGetErrorString(r));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(host, h->dens2, n * sizeof(double), hipMemcpyDeviceToHost));
+    return SKX_OK;
+  }
+  if (kind == SKX_DENSITY_POTENTIAL || kind == SKX_DENSITY_RHO) {
+    const int nf = h->L.nf;
+    if (kind == SKX_DENSITY_POTENTIAL && nf < 5) return fail(SKX_EINVAL, "theory has no potential component");
+    if (kind == SKX_DENSITY_RHO && nf < 13) return fail(SKX_EINVAL, "theory has no rho components");
+    std::vector<double> f((size_t)h->L.N * nf);
+    if (int rc = skx_download(h, SKX_FIELD, f.data(), f.size())) return rc;
+    const size_t N = (size_t)h->L.N;
+    if (kind == SKX_DENSITY_POTENTIAL) {
+      std::memcpy(host, f.data() + 4 * N, N * sizeof(double));
+    } else {
+      for (size_t i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int a = 4; a < nf; ++a) s += f[a * N + i] * f[a * N + i];
+        host[i] = s;
+      }
+    }
+    return SKX_OK;
+  }
+  return fail(SKX_EINVAL, "unknown density kind");
+}
+
+int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n) {
+  CHECK_HANDLE(h);
+  if (!temp) return fail(SKX_EINVAL, "null stage field");
+  if (n != (size_t)(h->L.N * h->L.nf)) return fail(SKX_EINVAL, "probe size must be nf*N");
+  if (int rc = ensure_eg(h)) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // the stage field goes to the l2 slot, the probe writes l1: solver F / V stay untouched
+  if (int rc = copy_in(h, h->l[1], temp, n)) return rc;
+  hipError_t r = launch(h, 1, 0, h->cur, h->eg, nullptr, nullptr, h->l[1]);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("probe: ") + hipGetErrorString(r));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (eg) HIP_TRY(hipMemcpy(eg, h->eg, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (l) {
+    if (int rc = copy_out(h, l, h->l[0], n)) return rc;
+  }
+  return SKX_OK;
+}
+
+int skx_history_attach(skx_handle h, int32_t capacity) {
+  CHECK_HANDLE(h);
+  if (capacity < 0) return fail(SKX_EINVAL, "capacity must be >= 0");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (h->hist) {
+    (void)hipFree(h->hist);
+    h->hist = nullptr;
+    h->hist_cap = 0;
+  }
+  drop_graphs(h);  // captured launches hold the old history pointer
+  if (capacity == 0) return SKX_OK;
+  if (hipMalloc(&h->hist, (size_t)capacity * sizeof(StepRecord)) != hipSuccess) {
+    h->hist = nullptr;
+    return fail(SKX_ENOMEM, "history allocation failed");
+  }
+  h->hist_cap = capacity;
+  HIP_TRY(launch_fill(h->hist, (size_t)capacity * sizeof(StepRecord), h->stream));
+  return SKX_OK;
+}
+
+int skx_history_read(skx_handle h, skx_step_record* out, int32_t n, int32_t* n_read) {
+  CHECK_HANDLE(h);
+  if (!out || !n_read) return fail(SKX_EINVAL, "null output");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  int k = std::min(n, std::min(h->hist_cap, h->st_host->steps));
+  if (k < 0) k = 0;
+  if (k > 0) HIP_TRY(hipMemcpy(out, h->hist, (size_t)k * sizeof(StepRecord), hipMemcpyDeviceToHost));
+  *n_read = k;
+  return SKX_OK;
+}
+
+int skx_trace_read(skx_handle h, double ms[5], int64_t counts[5]) {
+  CHECK_HANDLE(h);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  resolve_trace(h);
+  for (int i = 0; i < 5; ++i) {
+    if (ms) ms[i] = h->trace_ms[i];
+    if (counts) counts[i] = h->trace_n[i];
+  }
+  return SKX_OK;
+}
+
+int skx_trace_reset(skx_handle h) {
+  CHECK_HANDLE(h);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  resolve_trace(h);
+  for (int i = 0; i < 5; ++i) {
+    h->trace_ms[i] = 0;
+    h->trace_n[i] = 0;
+  }
+  return SKX_OK;
+}
+
+int skx_sync(skx_handle h) {
+  CHECK_HANDLE(h);
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return SKX_OK;
+}
+
+}  // extern "C"

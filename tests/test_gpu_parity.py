@@ -1,0 +1,177 @@
+"""GPU parity of the HIP engine against the FP64 oracle (through the C ABI).
+
+Tolerances: the FP64 engine must match the oracle to 1e-12 relative
+(norm-wise) on gradients, fields and energies; the FP32 form is checked
+against the FP64 oracle with the per-result tolerances stated in each test.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import make_params, rel_l2, smooth_field
+from oracle import integrator as OI
+from oracle import stencils as OS
+from oracle import theories as OT
+from skyrmion_solver_amd import _engine
+
+pytestmark = pytest.mark.gpu
+
+THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB}
+SHAPES = [(20, 18, 16), (9, 7, 13), (24, 40, 70)]
+
+
+def _engine_for(theory, rp, p_i, p_f, prec=_engine.FP64, **kw):
+    print(f"[case] theory={theory} lattice={rp.xlen}x{rp.ylen}x{rp.zlen} prec={prec} opts={kw}", flush=True)
+    return _engine.Engine(THEORY_IDS[theory], prec, p_i, p_f, **kw)
+
+
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_gradient_probe_fp64(theory, shape):
+    mod, rp, p_i, p_f = make_params(theory, *shape)
+    nf = int(p_i[5])
+    T = smooth_field(rp, nf, seed=17 * len(theory) + shape[0] + 3 * shape[2])
+    st = OI.OracleState(theory, p_i, p_f, T)
+    EG_ref, l_ref = OI.gradient(st, st.F)
+    with _engine_for(theory, rp, p_i, p_f) as eng:
+        eg, l = eng.eval_gradient(T)
+    assert rel_l2(eg, EG_ref) < 1e-12
+    assert rel_l2(l, l_ref) < 1e-12
+    # pointwise: every site within 1e-10 of the largest gradient entry
+    assert np.max(np.abs(eg - EG_ref.ravel())) <= 1e-10 * np.max(np.abs(EG_ref))
+
+
+@pytest.mark.parametrize("tile_z", [16, 32, 64])
+def test_gradient_probe_tiles(tile_z):
+    mod, rp, p_i, p_f = make_params("pion", 19, 21, 67)
+    T = smooth_field(rp, 4, seed=7)
+    st = OI.OracleState("pion", p_i, p_f, T)
+    EG_ref, _ = OI.gradient(st, st.F)
+    for xc in (3, 8, 19):
+        with _engine_for("pion", rp, p_i, p_f, tile_z=tile_z, x_chunk=xc) as eng:
+            eg, _ = eng.eval_gradient(T)
+        assert rel_l2(eg, EG_ref) < 1e-12, (tile_z, xc)
+
+
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+def test_energy_baryon_fp64(theory):
+    mod, rp, p_i, p_f = make_params(theory, 22, 20, 18)
+    nf = int(p_i[5])
+    F = smooth_field(rp, nf, seed=3, normalise=True)
+    st = OI.OracleState(theory, p_i, p_f, F)
+    with _engine_for(theory, rp, p_i, p_f) as eng:
+        eng.upload(_engine.FIELD, F)
+        E = eng.energy()
+        B = eng.baryon()
+        de = eng.density(_engine.DENSITY_ENERGY)
+        db = eng.density(_engine.DENSITY_BARYON)
+    E_ref = OI.energy(st)
+    B_ref = OI.baryon_number(st)
+    assert abs(E - E_ref) <= 1e-12 * abs(E_ref)
+    assert abs(B - B_ref) <= 1e-12 * max(1.0, abs(B_ref))
+    assert rel_l2(de, OI.energy_density(st)) < 1e-12
+    assert rel_l2(db, OI.baryon_density(st)) < 1e-12
+
+
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+def test_anf_steps_fp64(theory):
+    mod, rp, p_i, p_f = make_params(theory, 20, 18, 16)
+    nf = int(p_i[5])
+    F = smooth_field(rp, nf, seed=11, normalise=True)
+    st = OI.OracleState(theory, p_i, p_f, F)
+    E_prev = OI.energy(st)
+    with _engine_for(theory, rp, p_i, p_f) as eng:
+        eng.upload(_engine.FIELD, F)
+        eng.upload(_engine.VELOCITY, np.zeros_like(F))
+        Eg = E_prev
+        for k in range(6):
+            E_ref, err_ref = OI.anf_step(st, E_prev)
+            E_gpu, err_gpu, errabs_gpu = eng.anf_step(Eg)
+            assert abs(E_gpu - E_ref) <= 1e-12 * abs(E_ref), k
+            assert abs(err_gpu - err_ref) <= 1e-10 * max(abs(err_ref), 1e-300), k
+            assert abs(errabs_gpu - OI.max_abs_force(st)) <= 1e-10 * OI.max_abs_force(st), k
+            E_prev, Eg = E_ref, E_gpu
+        Fg = eng.download(_engine.FIELD)
+        Vg = eng.download(_engine.VELOCITY)
+    assert rel_l2(Fg, st.F) < 1e-12
+    assert rel_l2(Vg, st.V) < 1e-10 or np.max(np.abs(st.V)) == 0.0
+
+
+def test_arrest_applied_like_reference():
+    """Large dt makes the energy rise: the velocity must be zeroed exactly when E_new > E_prev."""
+    mod, rp, p_i, p_f = make_params("pion", 16, 16, 16, time_step=0.1)
+    F = smooth_field(rp, 4, seed=5, normalise=True)
+    st = OI.OracleState("pion", p_i, p_f, F)
+    with _engine_for("pion", rp, p_i, p_f) as eng:
+        eng.upload(_engine.FIELD, F)
+        E_prev = Eg = OI.energy(st)
+        arrests = 0
+        for k in range(12):
+            E_ref, _ = OI.anf_step(st, E_prev)
+            E_gpu, _, _ = eng.anf_step(Eg)
+            Vg = eng.download(_engine.VELOCITY)
+            if np.max(np.abs(st.V)) == 0.0:
+                arrests += 1
+                assert np.max(np.abs(Vg)) == 0.0
+            assert rel_l2(Vg, st.V) < 1e-9 or np.max(np.abs(st.V)) == 0.0
+            E_prev, Eg = E_ref, E_gpu
+    print("arrests:", arrests)
+
+
+@pytest.mark.parametrize("theory", ["pion", "coulomb"])
+def test_run_equals_steps_bitwise(theory):
+    """The device-resident loop (fused energy, deferred arrest, hipGraph) equals repeated step()."""
+    mod, rp, p_i, p_f = make_params(theory, 18, 17, 20, time_step=0.1)
+    nf = int(p_i[5])
+    F = smooth_field(rp, nf, seed=2, normalise=True)
+    with _engine_for(theory, rp, p_i, p_f) as a, _engine_for(theory, rp, p_i, p_f) as b:
+        a.upload(_engine.FIELD, F)
+        b.upload(_engine.FIELD, F)
+        E = a.energy()
+        for _ in range(9):
+            E, err, errabs = a.anf_step(E)
+        stats = b.anf_run(9, -1.0, 4)
+        assert stats.steps == 9
+        assert stats.energy == E
+        assert stats.err_signed == err
+        assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
+        assert np.array_equal(a.download(_engine.VELOCITY), b.download(_engine.VELOCITY))
+
+
+def test_run_no_graph_equals_graph():
+    mod, rp, p_i, p_f = make_params("pion", 16, 18, 20, time_step=0.1)
+    F = smooth_field(rp, 4, seed=9, normalise=True)
+    with _engine_for("pion", rp, p_i, p_f) as a, \
+            _engine_for("pion", rp, p_i, p_f, flags=_engine.FLAG_NO_GRAPH) as b:
+        a.upload(_engine.FIELD, F)
+        b.upload(_engine.FIELD, F)
+        sa = a.anf_run(11, -1.0, 3)
+        sb = b.anf_run(11, -1.0, 5)
+        assert sa.steps == sb.steps == 11
+        assert sa.energy == sb.energy
+        assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
+
+
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+def test_fp32_form_against_fp64_oracle(theory):
+    """FP32 storage/stencils: after 10 steps, |dE|/E <= 1e-5, |dB| <= 1e-4, field rel-L2 <= 1e-5."""
+    mod, rp, p_i, p_f = make_params(theory, 20, 18, 16)
+    nf = int(p_i[5])
+    F = smooth_field(rp, nf, seed=4, normalise=True)
+    st = OI.OracleState(theory, p_i, p_f, F)
+    E_prev = OI.energy(st)
+    with _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32) as eng:
+        eng.upload(_engine.FIELD, F)
+        Eg = E_prev
+        for _ in range(10):
+            E_ref, _ = OI.anf_step(st, E_prev)
+            Eg, _, _ = eng.anf_step(Eg)
+            E_prev = E_ref
+        Fg = eng.download(_engine.FIELD)
+        B = eng.baryon()
+    B_ref = OI.baryon_number(st)
+    assert abs(Eg - E_ref) <= 1e-5 * abs(E_ref)
+    assert abs(B - B_ref) <= 1e-4
+    assert rel_l2(Fg, st.F) <= 1e-5
