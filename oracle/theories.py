@@ -442,7 +442,7 @@ def energy_rho(phi, d1, p_i, p_f):
     mrho2 = mrho * mrho
     c_alpha = p_f[10]
     F = d1
-    energy = np.zeros_like(phi[0])
+    energy = 0.0 * phi[0]
     for i in range(3):
         for mu in range(4):
             energy = energy + F[i, mu] * F[i, mu]

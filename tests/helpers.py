@@ -14,7 +14,8 @@ THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model",
 def make_params(theory: str, X: int, Y: int, Z: int, **kw):
     """Resolved params + packed arrays for a small lattice (non-uniform spacings by default)."""
     mod = load_theory(THEORY_NAMES[theory])
-    base = dict(xlen=X, ylen=Y, zlen=Z, lsx=0.21, lsy=0.19, lsz=0.2, time_step=0.05)
+    # time_step ~ 0.15 * spacing keeps RK4 stable for the quartic term at these gradients
+    base = dict(xlen=X, ylen=Y, zlen=Z, lsx=0.21, lsy=0.19, lsz=0.2, time_step=0.03)
     if theory == "pion":
         base.update(mpi=0.7, kappa=1.1)
     if theory == "omega":
@@ -37,7 +38,7 @@ def smooth_field(rp, nf: int, seed: int = 0, noise: float = 0.02, normalise: boo
     Extra components (omega, V, rho) get smooth seeded bumps.
     """
     rng = np.random.default_rng(seed)
-    cfg = dict(mode="hedgehog", profile=("exp", 0.6), isorotation="haar", jitter=0.05, seed=seed)
+    cfg = dict(mode="hedgehog", profile=("exp", 0.9), isorotation="haar", jitter=0.05, seed=seed)
     F = initial_data.build_field(rp, nf, cfg)
     rx, ry, rz = initial_data.coordinates(rp)
     for a in range(nf):

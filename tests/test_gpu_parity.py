@@ -101,7 +101,7 @@ def test_anf_steps_fp64(theory):
 
 def test_arrest_applied_like_reference():
     """Large dt makes the energy rise: the velocity must be zeroed exactly when E_new > E_prev."""
-    mod, rp, p_i, p_f = make_params("pion", 16, 16, 16, time_step=0.1)
+    mod, rp, p_i, p_f = make_params("pion", 16, 16, 16, time_step=0.04)
     F = smooth_field(rp, 4, seed=5, normalise=True)
     st = OI.OracleState("pion", p_i, p_f, F)
     with _engine_for("pion", rp, p_i, p_f) as eng:
@@ -123,7 +123,7 @@ def test_arrest_applied_like_reference():
 @pytest.mark.parametrize("theory", ["pion", "coulomb"])
 def test_run_equals_steps_bitwise(theory):
     """The device-resident loop (fused energy, deferred arrest, hipGraph) equals repeated step()."""
-    mod, rp, p_i, p_f = make_params(theory, 18, 17, 20, time_step=0.1)
+    mod, rp, p_i, p_f = make_params(theory, 18, 17, 20, time_step=0.04)
     nf = int(p_i[5])
     F = smooth_field(rp, nf, seed=2, normalise=True)
     with _engine_for(theory, rp, p_i, p_f) as a, _engine_for(theory, rp, p_i, p_f) as b:
@@ -141,7 +141,7 @@ def test_run_equals_steps_bitwise(theory):
 
 
 def test_run_no_graph_equals_graph():
-    mod, rp, p_i, p_f = make_params("pion", 16, 18, 20, time_step=0.1)
+    mod, rp, p_i, p_f = make_params("pion", 16, 18, 20, time_step=0.04)
     F = smooth_field(rp, 4, seed=9, normalise=True)
     with _engine_for("pion", rp, p_i, p_f) as a, \
             _engine_for("pion", rp, p_i, p_f, flags=_engine.FLAG_NO_GRAPH) as b:
