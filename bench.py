@@ -116,21 +116,34 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--chunk", type=int, default=100,
+                    help="steps per captured graph; --steps should be a multiple so no launch is an early exit")
     ap.add_argument("--trace-steps", type=int, default=20)
     ap.add_argument("--solve-seeds", type=int, default=3)
     ap.add_argument("--solve-max-steps", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1.0e-4)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
+                    help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
         sys.exit(2)
 
+    if args.only == "copy":
+        gbs, ms = _engine.stream_copy_bandwidth(1 << 30, 20)
+        print(f"[bench] stream copy of 1 GiB: {gbs:.1f} GB/s, {ms:.4f} ms per copy", file=sys.stderr, flush=True)
+        return
     rp, p_i, p_f, F = workload()
     N = int(p_i[6])
     nf = 4
+    if args.only in ("fp32", "fp64"):
+        prec = _engine.FP32 if args.only == "fp32" else _engine.FP64
+        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk)
+        print(f"[bench] {args.only}: {N * r['steps'] / r['seconds']:.4g} site-updates/s", file=sys.stderr, flush=True)
+        return
+    copy_gbs, _ = _engine.stream_copy_bandwidth(1 << 30, 20)
     res = {}
     for name, prec in (("fp64", _engine.FP64), ("fp32", _engine.FP32)):
         r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk)
@@ -152,7 +165,8 @@ def main():
     achieved = bytes_per_launch / (kern[dom]["avg_us"] * 1e-6) / 1e9
     roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=None,
-                    bytes_per_launch=bytes_per_launch)
+                    bytes_per_launch=bytes_per_launch, measured_copy_GBs=copy_gbs,
+                    frac_of_copy=achieved / copy_gbs)
 
     latency = {}
     if args.solve_seeds > 0:

@@ -159,6 +159,12 @@ int skx_trace_reset(skx_handle h);
 /* Wait for all work queued on the handle's stream. */
 int skx_sync(skx_handle h);
 
+/* Diagnostic (no reference counterpart): time `reps` launches of a plain
+ * streaming copy of `bytes` bytes on `device`; returns read+write GB/s and
+ * milliseconds per copy.  Used as the measured copy peak and to calibrate
+ * HBM byte counters against a kernel of known traffic. */
+int skx_diag_stream_copy(int32_t device, size_t bytes, int32_t reps, double* gbs, double* ms_per_copy);
+
 const char* skx_last_error(void);
 int skx_version(void);
 

@@ -15,7 +15,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libskx.so"
-SOURCES = ["skx_kernels.hip", "skx_runtime.hip"]
+SOURCES = ["skx_kernels.hip", "skx_runtime.hip", "skx_diag.hip"]
 ARCH = os.environ.get("SKX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -35,7 +35,7 @@ EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
     "skx_anf_run", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density", "skx_eval_gradient",
     "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_sync",
-    "skx_last_error", "skx_version",
+    "skx_diag_stream_copy", "skx_last_error", "skx_version",
 )
 
 
@@ -105,6 +105,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_trace_read": ([H, dp, C.POINTER(C.c_int64)], C.c_int),
         "skx_trace_reset": ([H], C.c_int),
         "skx_sync": ([H], C.c_int),
+        "skx_diag_stream_copy": ([C.c_int32, C.c_size_t, C.c_int32, dp, dp], C.c_int),
         "skx_last_error": ([], C.c_char_p),
         "skx_version": ([], C.c_int),
     }
@@ -145,6 +146,14 @@ def footprint_bytes(theory: int, precision: int, p_i, p_f, tile_z: int = 0, x_ch
     out = C.c_size_t(0)
     _check(lib.skx_footprint_bytes(C.byref(d), C.byref(out)), "skx_footprint_bytes")
     return int(out.value)
+
+
+def stream_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20, device: int = 0):
+    """Measured copy bandwidth (GB/s, read + write) and ms per copy of ``nbytes``."""
+    lib = load_library()
+    g, ms = C.c_double(), C.c_double()
+    _check(lib.skx_diag_stream_copy(device, nbytes, reps, C.byref(g), C.byref(ms)), "skx_diag_stream_copy")
+    return g.value, ms.value
 
 
 class Engine:

@@ -75,6 +75,18 @@ struct StageArgs {
   int flags;
 };
 
+// One lattice of a batched launch.  args[parity] are the stage arguments for
+// ping-pong parity 0 / 1 (F, V read from buffer `parity`, written to the
+// other); the lattice owns blocks [first_block, first_block + nblocks[stage]).
+template <typename T>
+struct BatchSlot {
+  StageArgs<T> args[2];
+  int first_block;
+  int nblocks[5];
+  int active;
+  int pad_;
+};
+
 // Kernel launchers (skx_kernels.hip).  Return hipSuccess or the launch error.
 struct LaunchShape {
   int tz, ty, xchunk;
@@ -84,6 +96,13 @@ hipError_t launch_stage_f64(int theory, int stage, const StageArgs<double>& a, c
 hipError_t launch_stage_f32(int theory, int stage, const StageArgs<float>& a, const LaunchShape& s, hipStream_t st);
 int stage_blocks(int X, int Y, int Z, const LaunchShape& s);
 bool shape_supported(const LaunchShape& s);
+// resident blocks per CU of the stage kernel (hipOccupancyMaxActiveBlocksPerMultiprocessor)
+hipError_t stage_blocks_per_cu(int prec_fp32, int theory, int stage, const LaunchShape& s, int* blocks);
+
+hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, const BatchSlot<double>* slots,
+                                  const int* blk_inst, int parity, int grid, hipStream_t st);
+hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, const BatchSlot<float>* slots,
+                                  const int* blk_inst, int parity, int grid, hipStream_t st);
 
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);  // zero fill kernel (graph safe)
 hipError_t launch_apply_arrest_f64(double* V, long long n, const DevStatus* st, hipStream_t s);

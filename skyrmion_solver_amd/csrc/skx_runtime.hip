@@ -9,6 +9,7 @@
 // a captured hipGraph of a chunk of steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -101,20 +102,23 @@ int parse_desc(const skx_desc* d, Lattice& L) {
   return SKX_OK;
 }
 
-LaunchShape choose_shape(const Lattice& L, const skx_desc* d) {
-  LaunchShape s{32, 8, 16};
-  if (d->tile_z == 64) s = {64, 4, 16};
-  if (d->tile_z == 16) s = {16, 16, 16};
-  if (d->tile_z == 0 && L.Z <= 16) s = {16, 16, 16};
-  if (d->x_chunk > 0) s.xchunk = d->x_chunk;
-  else {
-    // enough blocks to fill 256 CUs several times over, but chunks no shorter than 8 planes
-    const int cols = ((L.Z + s.tz - 1) / s.tz) * ((L.Y + s.ty - 1) / s.ty);
-    int xc = L.X;
-    while (xc > 8 && (long long)cols * ((L.X + xc - 1) / xc) < 2048) xc = (xc + 1) / 2;
-    s.xchunk = xc;
-  }
+LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
+  LaunchShape s{32, 8, 1};
+  if (d->tile_z == 64) s = {64, 4, 1};
+  if (d->tile_z == 16) s = {16, 16, 1};
+  if (d->tile_z == 0 && L.Z <= 16) s = {16, 16, 1};
   return s;
+}
+
+// Planes per block along x for one stage kernel: as many x-chunks as keep
+// every block of the launch resident at once (CUs x resident blocks per CU),
+// since a block's run time grows with (chunk + 4) halo-inclusive planes.
+int choose_xchunk(const Lattice& L, const LaunchShape& tile, int blocks_per_cu, int n_cu, int forced) {
+  if (forced > 0) return forced;
+  const long long cols = (long long)((L.Z + tile.tz - 1) / tile.tz) * ((L.Y + tile.ty - 1) / tile.ty);
+  const long long capacity = (long long)std::max(blocks_per_cu, 1) * std::max(n_cu, 1);
+  const long long nchunks = std::max(1LL, capacity / std::max(cols, 1LL));
+  return (int)std::max(1LL, (L.X + nchunks - 1) / nchunks);
 }
 
 size_t elem_size(int prec) { return prec == SKX_FP64 ? sizeof(double) : sizeof(float); }
@@ -125,7 +129,7 @@ struct skx_engine {
   Lattice L;
   int device = 0;
   uint32_t flags = 0;
-  LaunchShape shape{};
+  LaunchShape shape[5]{};  // per stage kind: 0 energy pass, 1..4 RK4 stages
   hipStream_t stream = nullptr;
   void* F[2] = {nullptr, nullptr};
   void* V[2] = {nullptr, nullptr};
@@ -155,10 +159,16 @@ struct skx_engine {
 
 namespace {
 
-size_t footprint(const Lattice& L, const LaunchShape& s) {
+// upper bound of the blocks of any stage launch (x-chunks of one plane)
+size_t max_blocks(const Lattice& L, const LaunchShape& tile) {
+  LaunchShape s = tile;
+  s.xchunk = 1;
+  return (size_t)stage_blocks(L.X, L.Y, L.Z, s);
+}
+
+size_t footprint(const Lattice& L, const LaunchShape& tile) {
   const size_t arr = (size_t)L.N * L.nf * elem_size(L.prec);
-  const size_t nblk = (size_t)stage_blocks(L.X, L.Y, L.Z, s);
-  return 7 * arr + 2 * nblk * sizeof(double) + sizeof(DevStatus);
+  return 7 * arr + 2 * max_blocks(L, tile) * sizeof(double) + sizeof(DevStatus);
 }
 
 template <typename T>
@@ -196,7 +206,7 @@ StageArgs<T> make_args(skx_engine* e, int stage, int flags, int c, const void* f
   a.Y = L.Y;
   a.Z = L.Z;
   a.halo = L.halo;
-  a.xchunk = e->shape.xchunk;
+  a.xchunk = e->shape[stage].xchunk;
   a.N = L.N;
   a.ilsx = T(1.0 / L.lsx);
   a.ilsy = T(1.0 / L.lsy);
@@ -247,13 +257,13 @@ hipError_t launch(skx_engine* e, int stage, int flags, int c, double* eg_out = n
     a.eg_out = eg_out;
     a.dens_e = dens_e;
     a.dens_b = dens_b;
-    r = launch_stage_f64(e->L.theory, stage, a, e->shape, e->stream);
+    r = launch_stage_f64(e->L.theory, stage, a, e->shape[stage], e->stream);
   } else {
     StageArgs<float> a = make_args<float>(e, stage, flags, c, field_override);
     a.eg_out = eg_out;
     a.dens_e = dens_e;
     a.dens_b = dens_b;
-    r = launch_stage_f32(e->L.theory, stage, a, e->shape, e->stream);
+    r = launch_stage_f32(e->L.theory, stage, a, e->shape[stage], e->stream);
   }
   if (trace) {
     (void)hipEventRecord(e1, e->stream);
@@ -450,7 +460,7 @@ int skx_footprint_bytes(const skx_desc* desc, size_t* bytes) {
   Lattice L;
   if (int rc = parse_desc(desc, L)) return rc;
   if (!bytes) return fail(SKX_EINVAL, "null output");
-  *bytes = footprint(L, choose_shape(L, desc));
+  *bytes = footprint(L, choose_tile(L, desc));
   return SKX_OK;
 }
 
@@ -465,8 +475,8 @@ int skx_create(const skx_desc* desc, skx_handle* out) {
   e->L = L;
   e->device = desc->device;
   e->flags = desc->flags;
-  e->shape = choose_shape(L, desc);
-  if (!shape_supported(e->shape)) {
+  const LaunchShape tile = choose_tile(L, desc);
+  if (!shape_supported(tile)) {
     delete e;
     return fail(SKX_EINVAL, "unsupported launch shape");
   }
@@ -474,6 +484,20 @@ int skx_create(const skx_desc* desc, skx_handle* out) {
   if (r != hipSuccess) {
     delete e;
     return fail(SKX_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(r));
+  }
+  int n_cu = 0;
+  if ((r = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, e->device)) != hipSuccess) {
+    delete e;
+    return fail(SKX_EHIP, std::string("hipDeviceGetAttribute: ") + hipGetErrorString(r));
+  }
+  for (int stg = 0; stg < 5; ++stg) {
+    int bpc = 0;
+    if ((r = stage_blocks_per_cu(L.prec == SKX_FP32, L.theory, stg, tile, &bpc)) != hipSuccess) {
+      delete e;
+      return fail(SKX_EHIP, std::string("occupancy query: ") + hipGetErrorString(r));
+    }
+    e->shape[stg] = tile;
+    e->shape[stg].xchunk = choose_xchunk(L, tile, bpc, n_cu, desc->x_chunk);
   }
   if ((r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
     e->stream = nullptr;
@@ -489,7 +513,8 @@ int skx_create(const skx_desc* desc, skx_handle* out) {
       return fail(SKX_ENOMEM, "device allocation of lattice arrays failed");
     }
   }
-  const size_t nblk = (size_t)stage_blocks(L.X, L.Y, L.Z, e->shape);
+  size_t nblk = 0;
+  for (int stg = 0; stg < 5; ++stg) nblk = std::max(nblk, (size_t)stage_blocks(L.X, L.Y, L.Z, e->shape[stg]));
   if (hipMalloc(&e->partial, 2 * nblk * sizeof(double)) != hipSuccess ||
       hipMalloc(&e->st, sizeof(DevStatus)) != hipSuccess ||
       hipHostMalloc(&e->st_host, sizeof(DevStatus), hipHostMallocDefault) != hipSuccess) {

@@ -1,0 +1,126 @@
+"""Pin the oracle's RK4 / arrested-Newton-flow step (core/integrator.py:284-410).
+
+* On a linear problem (g(phi) = w^2 phi) one step equals the closed-form RK4
+  amplification exp_4(dt A) of the first-order system (phi, V)' = (V, -w^2 phi);
+* after a step with unit_length the S^3 components are normalised and V is
+  tangent; the vacuum is a fixed point;
+* the arrest rule zeroes V exactly when E_new > E_prev (strict) and only if
+  killkinen is set.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from helpers import make_params, smooth_field
+from oracle import integrator as OI
+from oracle import reductions as OR
+
+
+def test_rk4_linear_amplification_closed_form():
+    rng = np.random.default_rng(0)
+    w2, dt = 3.7, 0.11
+    F = rng.normal(size=(4, 5, 6, 7))
+    V = rng.normal(size=F.shape)
+    F0, V0 = F.copy(), V.copy()
+
+    def grad(T):
+        EG = w2 * T
+        return EG, -dt * EG
+
+    OI.rk4_step(F, V, dt, grad)
+    A = np.array([[0.0, 1.0], [-w2, 0.0]])
+    M = np.eye(2)
+    term = np.eye(2)
+    for k in range(1, 5):
+        term = term @ (dt * A) / k
+        M = M + term
+    np.testing.assert_allclose(F, M[0, 0] * F0 + M[0, 1] * V0, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(V, M[1, 0] * F0 + M[1, 1] * V0, rtol=1e-13, atol=1e-13)
+
+
+def test_rk4_planted_stage_error_breaks_closed_form(monkeypatch):
+    """A wrong stage weight (k2 with f = 0.4) must violate the amplification check."""
+    rng = np.random.default_rng(1)
+    w2, dt = 2.0, 0.2
+    F = rng.normal(size=(4, 3, 3, 3))
+    V = rng.normal(size=F.shape)
+    F0, V0 = F.copy(), V.copy()
+
+    def grad(T):
+        EG = w2 * T
+        return EG, -dt * EG
+
+    def bad_rk4(F, V, dt, gradient_fn, finalise_fn=None):
+        k1 = dt * V
+        EG, l1 = gradient_fn(F)
+        k2 = dt * (V + 0.5 * l1)
+        EG, l2 = gradient_fn(F + 0.4 * k1)
+        k3 = dt * (V + 0.5 * l2)
+        EG, l3 = gradient_fn(F + 0.5 * k2)
+        k4 = dt * (V + l3)
+        EG, l4 = gradient_fn(F + k3)
+        V += (l1 + 2 * l2 + 2 * l3 + l4) / 6
+        F += (k1 + 2 * k2 + 2 * k3 + k4) / 6
+        return EG
+
+    bad_rk4(F, V, dt, grad)
+    A = np.array([[0.0, 1.0], [-w2, 0.0]])
+    M = sum(np.linalg.matrix_power(dt * A, k) / math.factorial(k) for k in range(5))
+    assert np.abs(F - (M[0, 0] * F0 + M[0, 1] * V0)).max() > 1e-6
+
+
+def test_step_keeps_s3_and_tangency():
+    mod, rp, p_i, p_f = make_params("pion", 14, 13, 12)
+    F = smooth_field(rp, 4, seed=1, normalise=True)
+    st = OI.OracleState("pion", p_i, p_f, F)
+    E = OI.energy(st)
+    for _ in range(3):
+        E, _ = OI.anf_step(st, E)
+    np.testing.assert_allclose(np.sum(st.F[:4] ** 2, axis=0), 1.0, atol=1e-14)
+    assert np.abs(np.sum(st.V[:4] * st.F[:4], axis=0)).max() < 1e-14
+
+
+def test_vacuum_is_fixed_by_a_step():
+    mod, rp, p_i, p_f = make_params("pion", 10, 10, 10)
+    F = np.zeros((4, 10, 10, 10))
+    F[0] = 1.0
+    st = OI.OracleState("pion", p_i, p_f, F)
+    E, err = OI.anf_step(st, 0.0)
+    # the reference's left-to-right stencil sum leaves O(1e-17 / ls) first
+    # differences on a constant field, so E is a rounding-level positive number
+    assert 0.0 <= E < 1e-25 and abs(err) < 1e-12
+    assert np.array_equal(st.F, F) and np.max(np.abs(st.V)) == 0.0
+
+
+def test_arrest_rule_strict_and_gated():
+    mod, rp, p_i, p_f = make_params("pion", 12, 12, 12)
+    F = smooth_field(rp, 4, seed=2, normalise=True)
+    st = OI.OracleState("pion", p_i, p_f, F)
+    st.V[:] = 0.01
+    E0 = OI.energy(st)
+    # prev energy far below anything reachable: E_new > E_prev -> V zeroed
+    OI.anf_step(st, -1.0e9)
+    assert np.max(np.abs(st.V)) == 0.0
+    # prev energy above: V kept
+    st2 = OI.OracleState("pion", p_i, p_f, F)
+    st2.V[:] = 0.01
+    OI.anf_step(st2, 1.0e9)
+    assert np.max(np.abs(st2.V)) > 0.0
+    # killkinen off: never arrests
+    p_i3 = p_i.copy()
+    p_i3[8] = 0
+    st3 = OI.OracleState("pion", p_i3, p_f, F)
+    st3.V[:] = 0.01
+    OI.anf_step(st3, -1.0e9)
+    assert np.max(np.abs(st3.V)) > 0.0
+
+
+def test_reductions_exact_and_signed():
+    v = np.arange(1, 5001, dtype=np.float64)
+    assert OR.ref_sum(v) == 5000 * 5001 / 2
+    assert OR.ref_max(-v) == -1.0
+    assert OR.ref_max(np.array([-3.0, -7.0])) == -3.0
+    assert OR.ref_sum(np.zeros(1025)) == 0.0
