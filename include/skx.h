@@ -159,6 +159,46 @@ int skx_trace_reset(skx_handle h);
 /* Wait for all work queued on the handle's stream. */
 int skx_sync(skx_handle h);
 
+/* ---------------------------------------------------------------- batches
+ * Many lattices of one theory / precision minimised together: one launch
+ * per RK4 stage covers every lattice (no reference counterpart; the
+ * reference runs one Simulation at a time).  Lattices join and leave between
+ * run calls; each keeps the exact per-lattice semantics of skx_anf_run. */
+typedef struct skx_batch* skx_batch_handle;
+
+typedef struct skx_batch_desc {
+  int32_t theory;       /* enum skx_theory (pion, omega or Coulomb) */
+  int32_t precision;    /* enum skx_precision */
+  int32_t device;
+  uint32_t flags;       /* SKX_FLAG_NO_GRAPH */
+  int32_t tile_z;       /* 16, 32 (default, 0) or 64 */
+  int32_t max_slots;    /* lattices resident at once */
+  int32_t max_blocks;   /* blocks of every batched launch (sum over lattices) */
+} skx_batch_desc;
+
+typedef struct skx_slot_status {
+  int32_t slot, active, steps, done, converged, nonfinite;
+  double energy;        /* energy of the last field reduced (final field after skx_batch_remove) */
+  double err_signed, err_abs, baryon;
+} skx_slot_status;
+
+int skx_batch_create(const skx_batch_desc* desc, skx_batch_handle* out);
+int skx_batch_destroy(skx_batch_handle b);
+/* Add a lattice (F0 / V0 host FP64, V0 may be NULL); it runs until its
+ * stage-4 max|g| < tol or max_steps.  SKX_EBUSY when no slot / blocks are free. */
+int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0, const double* V0, int32_t max_steps,
+                  double tol, int32_t* slot);
+/* Advance every active lattice by `steps` ANF steps (rounded up to even).
+ * skx_batch_run = skx_batch_launch (enqueue on the batch's stream, return at
+ * once) + skx_batch_wait; batches on different streams overlap on the device. */
+int skx_batch_run(skx_batch_handle b, int32_t steps);
+int skx_batch_launch(skx_batch_handle b, int32_t steps);
+int skx_batch_wait(skx_batch_handle b);
+int skx_batch_poll(skx_batch_handle b, skx_slot_status* out, int32_t n, int32_t* n_out);
+/* Finish a lattice (final energy, pending arrest) and copy its field and
+ * velocity out (either may be NULL); frees its slot. */
+int skx_batch_remove(skx_batch_handle b, int32_t slot, double* F_out, double* V_out, skx_slot_status* final_status);
+
 /* Diagnostic (no reference counterpart): time `reps` launches of a plain
  * streaming copy of `bytes` bytes on `device`; returns read+write GB/s and
  * milliseconds per copy.  Used as the measured copy peak and to calibrate

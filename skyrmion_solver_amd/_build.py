@@ -15,7 +15,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libskx.so"
-SOURCES = ["skx_kernels.hip", "skx_runtime.hip", "skx_diag.hip"]
+SOURCES = ["skx_kernels.hip", "skx_runtime.hip", "skx_diag.hip"] + [
+    f"skx_stage_{th}_{p}.hip" for th in ("pion", "omega", "coulomb", "rho") for p in ("f32", "f64")]
 ARCH = os.environ.get("SKX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -25,6 +26,10 @@ HIPCC_FLAGS = [
     "-fPIC",
     # packed FP32 instructions stay off: all FP32 math is scalar per lane
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b+c is fused exactly where one source expression says so, never by
+    # the backend: every instantiation of a stage body (solo, batched) then
+    # rounds identically, so a batched lattice is bit-identical to its solo run
+    "-ffp-contract=on",
 ]
 
 

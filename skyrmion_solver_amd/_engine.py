@@ -35,7 +35,9 @@ EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
     "skx_anf_run", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density", "skx_eval_gradient",
     "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_sync",
-    "skx_diag_stream_copy", "skx_last_error", "skx_version",
+    "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
+    "skx_batch_wait", "skx_batch_poll",
+    "skx_batch_remove", "skx_diag_stream_copy", "skx_last_error", "skx_version",
 )
 
 
@@ -63,6 +65,20 @@ class RunStats(C.Structure):
         ("err_signed", C.c_double), ("err_abs", C.c_double), ("baryon", C.c_double),
         ("seconds", C.c_double),
     ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("theory", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32), ("flags", C.c_uint32),
+                ("tile_z", C.c_int32), ("max_slots", C.c_int32), ("max_blocks", C.c_int32)]
+
+
+class SlotStatus(C.Structure):
+    _fields_ = [("slot", C.c_int32), ("active", C.c_int32), ("steps", C.c_int32), ("done", C.c_int32),
+                ("converged", C.c_int32), ("nonfinite", C.c_int32), ("energy", C.c_double),
+                ("err_signed", C.c_double), ("err_abs", C.c_double), ("baryon", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -105,6 +121,14 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_trace_read": ([H, dp, C.POINTER(C.c_int64)], C.c_int),
         "skx_trace_reset": ([H], C.c_int),
         "skx_sync": ([H], C.c_int),
+        "skx_batch_create": ([C.POINTER(BatchDesc), C.POINTER(H)], C.c_int),
+        "skx_batch_destroy": ([H], C.c_int),
+        "skx_batch_add": ([H, C.POINTER(Desc), dp, dp, C.c_int32, C.c_double, C.POINTER(C.c_int32)], C.c_int),
+        "skx_batch_run": ([H, C.c_int32], C.c_int),
+        "skx_batch_launch": ([H, C.c_int32], C.c_int),
+        "skx_batch_wait": ([H], C.c_int),
+        "skx_batch_poll": ([H, C.POINTER(SlotStatus), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "skx_batch_remove": ([H, C.c_int32, dp, dp, C.POINTER(SlotStatus)], C.c_int),
         "skx_diag_stream_copy": ([C.c_int32, C.c_size_t, C.c_int32, dp, dp], C.c_int),
         "skx_last_error": ([], C.c_char_p),
         "skx_version": ([], C.c_int),
@@ -263,3 +287,71 @@ class Engine:
 
     def sync(self):
         _check(self._lib.skx_sync(self._h), "skx_sync")
+
+
+class Batch:
+    """Many lattices of one theory / precision advanced by shared launches (``skx_batch_*``)."""
+
+    def __init__(self, theory: int, precision: int, max_slots: int = 64, max_blocks: int = 16384,
+                 tile_z: int = 32, device: int = 0, flags: int = 0):
+        self._lib = load_library()
+        d = BatchDesc(theory=theory, precision=precision, device=device, flags=flags, tile_z=tile_z,
+                      max_slots=max_slots, max_blocks=max_blocks)
+        h = C.c_void_p()
+        _check(self._lib.skx_batch_create(C.byref(d), C.byref(h)), "skx_batch_create")
+        self._h = h
+        self.theory, self.precision, self.max_slots = theory, precision, max_slots
+        self._shape = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.skx_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, p_i, p_f, F0, V0=None, max_steps: int = 100000, tol: float = -1.0, device: int = 0) -> int:
+        """Add a lattice; raises ``SkxError`` (code -5, SKX_EBUSY) when the batch is full."""
+        d, keep = make_desc(self.theory, self.precision, p_i, p_f, device)
+        f = np.ascontiguousarray(F0, dtype=np.float64).ravel()
+        v = None if V0 is None else np.ascontiguousarray(V0, dtype=np.float64).ravel()
+        slot = C.c_int32()
+        _check(self._lib.skx_batch_add(self._h, C.byref(d), _dptr(f), _dptr(v) if v is not None else None,
+                                       int(max_steps), float(tol), C.byref(slot)), "skx_batch_add")
+        self._shape[slot.value] = f.size
+        return slot.value
+
+    def run(self, steps: int):
+        _check(self._lib.skx_batch_run(self._h, int(steps)), "skx_batch_run")
+
+    def launch(self, steps: int):
+        """Enqueue `steps` steps and return at once (pair with :meth:`wait`)."""
+        _check(self._lib.skx_batch_launch(self._h, int(steps)), "skx_batch_launch")
+
+    def wait(self):
+        _check(self._lib.skx_batch_wait(self._h), "skx_batch_wait")
+
+    def poll(self) -> list:
+        buf = (SlotStatus * self.max_slots)()
+        n = C.c_int32()
+        _check(self._lib.skx_batch_poll(self._h, buf, self.max_slots, C.byref(n)), "skx_batch_poll")
+        return [buf[i].as_dict() for i in range(n.value)]
+
+    def remove(self, slot: int, want_fields: bool = True):
+        n = self._shape.pop(slot)
+        F = np.empty(n) if want_fields else None
+        V = np.empty(n) if want_fields else None
+        st = SlotStatus()
+        _check(self._lib.skx_batch_remove(self._h, int(slot), _dptr(F) if want_fields else None,
+                                          _dptr(V) if want_fields else None, C.byref(st)), "skx_batch_remove")
+        return st.as_dict(), F, V

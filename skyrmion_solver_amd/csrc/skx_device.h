@@ -1,0 +1,174 @@
+// Device helpers shared by the stage kernels: reductions, stencil
+// coefficients and buffer addressing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "skx_internal.h"
+
+namespace skx {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ double load_partial(const double* p) {
+  unsigned long long u =
+      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(u));
+}
+
+// Two-value block reduction, then the last block to finish folds every
+// block's partials in a fixed order (deterministic, no memset needed).
+// IS_MAX selects max instead of (Neumaier-compensated) sum.  Returns true in
+// thread 0 of the last block, with the grid-wide results in r0 / r1.
+template <int NT, bool IS_MAX>
+__device__ bool grid_reduce2(double& r0, double& r1, double* partial, uint32_t* counter, double* red, int blk,
+                             int nblk) {
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  double a = IS_MAX ? wave_max(r0) : wave_sum(r0);
+  double b = IS_MAX ? wave_max(r1) : wave_sum(r1);
+  if (lane == 0) {
+    red[w] = a;
+    red[NW + w] = b;
+  }
+  __syncthreads();
+  __shared__ bool is_last;
+  if (tid == 0) {
+    double s0 = red[0], s1 = red[NW];
+    for (int i = 1; i < NW; ++i) {
+      s0 = IS_MAX ? fmax(s0, red[i]) : s0 + red[i];
+      s1 = IS_MAX ? fmax(s1, red[NW + i]) : s1 + red[NW + i];
+    }
+    partial[2 * blk] = s0;
+    partial[2 * blk + 1] = s1;
+    __threadfence();
+    const uint32_t prev = atomicAdd(counter, 1u);
+    is_last = (prev == (uint32_t)nblk - 1u);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  const int nb = nblk;
+  double s0 = IS_MAX ? -1.0e300 : 0.0, c0 = 0.0;
+  double s1 = IS_MAX ? -1.0e300 : 0.0, c1 = 0.0;
+  for (int i = tid; i < nb; i += NT) {
+    const double v0 = load_partial(partial + 2 * i);
+    const double v1 = load_partial(partial + 2 * i + 1);
+    if (IS_MAX) {
+      s0 = fmax(s0, v0);
+      s1 = fmax(s1, v1);
+    } else {  // Neumaier compensated summation
+      double t = s0 + v0;
+      c0 += (fabs(s0) >= fabs(v0)) ? ((s0 - t) + v0) : ((v0 - t) + s0);
+      s0 = t;
+      t = s1 + v1;
+      c1 += (fabs(s1) >= fabs(v1)) ? ((s1 - t) + v1) : ((v1 - t) + s1);
+      s1 = t;
+    }
+  }
+  if (!IS_MAX) {
+    s0 += c0;
+    s1 += c1;
+  }
+  __syncthreads();
+  a = IS_MAX ? wave_max(s0) : wave_sum(s0);
+  b = IS_MAX ? wave_max(s1) : wave_sum(s1);
+  if (lane == 0) {
+    red[w] = a;
+    red[NW + w] = b;
+  }
+  __syncthreads();
+  if (tid != 0) return false;
+  s0 = red[0];
+  s1 = red[NW];
+  for (int i = 1; i < NW; ++i) {
+    s0 = IS_MAX ? fmax(s0, red[i]) : s0 + red[i];
+    s1 = IS_MAX ? fmax(s1, red[NW + i]) : s1 + red[NW + i];
+  }
+  *counter = 0u;  // ready for the next launch
+  r0 = s0;
+  r1 = s1;
+  return true;
+}
+
+template <int K>
+using IC = std::integral_constant<int, K>;
+
+// Stencil coefficients of one axis with the inverse spacing folded in.
+template <typename T>
+struct AxisCoef {
+  T d1a, d1b;       // D1 = d1a (f[-2] - f[+2]) + d1b (f[+1] - f[-1])
+  T d2a, d2b, d2c;  // D2 = d2a (f[-1] + f[+1]) - d2b (f[-2] + f[+2]) - d2c f[0]
+  __device__ void set(T ils) {
+    d1a = T(1.0 / 12.0) * ils;
+    d1b = T(2.0 / 3.0) * ils;
+    const T i2 = ils * ils;
+    d2a = T(4.0 / 3.0) * i2;
+    d2b = T(1.0 / 12.0) * i2;
+    d2c = T(2.5) * i2;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T cd1(T fm2, T fm1, T fp1, T fp2, T a, T b) {
+  return a * (fm2 - fp2) + b * (fp1 - fm1);
+}
+template <typename T>
+__device__ __forceinline__ T cd2(T fm2, T fm1, T f0, T fp1, T fp2, T a, T b, T c) {
+  return a * (fm1 + fp1) - b * (fm2 + fp2) - c * f0;
+}
+
+// RK4 stage field at one site, formed on load from the field f, the velocity
+// v (ignored while the arrest holds, keep == 0) and the previous stage's
+// l = -dt g (core/integrator.py:382-396: Temp = F + f_s k_{s-1},
+// k_s = dt (V + f_s l_{s-1}), f = 0, 1/2, 1/2, 1).
+template <int STAGE, typename T>
+__device__ __forceinline__ T stage_value(T f, T v, T l, T dt, T keep) {
+  const T vk = keep != T(0) ? v : T(0);
+  if constexpr (STAGE == 2) return f + T(0.5) * (dt * vk);
+  else if constexpr (STAGE == 3) return f + T(0.5) * (dt * (vk + T(0.5) * l));
+  else if constexpr (STAGE == 4) return f + dt * (vk + T(0.5) * l);
+  else return f;
+}
+
+// Buffer addressing: one resource per (array, component, plane) with the
+// plane's byte size as the range, so every load / store takes a 32-bit lane
+// offset and the scalar base; offsets at or beyond the range (OOB) read 0 and
+// drop stores in hardware, which is how halo cells outside the lattice are
+// zeroed.
+constexpr uint32_t kOOB = 0x80000000u;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, 0, 0);
+  } else {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+  }
+}
+
+
+}  // namespace skx

@@ -59,6 +59,7 @@ struct StageArgs {
   T* lout;       // stages 1-3: l_s
   T* Fout;       // stage 4: new field
   T* Vout;       // stage 4: new velocity
+  T* aux;        // rho: per-site D^a_j (9 components) then R^a_ij (9), written by the curl pass
   double* eg_out;   // optional: projected gradient (double) of this stage
   double* dens_e;   // optional (stage 0): energy density per site
   double* dens_b;   // optional (stage 0): baryon density per site
@@ -75,14 +76,15 @@ struct StageArgs {
   int flags;
 };
 
-// One lattice of a batched launch.  args[parity] are the stage arguments for
-// ping-pong parity 0 / 1 (F, V read from buffer `parity`, written to the
-// other); the lattice owns blocks [first_block, first_block + nblocks[stage]).
+// One lattice of a batched launch.  args[s-1][parity] are the arguments of
+// RK4 stage s for ping-pong parity 0 / 1 (F, V read from buffer `parity`,
+// written to the other); the lattice owns blocks
+// [first_block, first_block + nblocks) of every batched stage launch.
 template <typename T>
 struct BatchSlot {
-  StageArgs<T> args[2];
+  StageArgs<T> args[4][2];
   int first_block;
-  int nblocks[5];
+  int nblocks;
   int active;
   int pad_;
 };
@@ -103,6 +105,15 @@ hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, c
                                   const int* blk_inst, int parity, int grid, hipStream_t st);
 hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, const BatchSlot<float>* slots,
                                   const int* blk_inst, int parity, int grid, hipStream_t st);
+
+// Per-(precision, theory) launchers, instantiated in skx_stage_<theory>_<precision>.hip.
+template <typename T, int TH>
+hipError_t stage_launch(int stage, const StageArgs<T>& a, const LaunchShape& s, hipStream_t st);
+template <typename T, int TH>
+hipError_t stage_launch_batch(int stage, const LaunchShape& s, const BatchSlot<T>* slots, const int* blk_inst,
+                              int parity, int grid, hipStream_t st);
+template <typename T, int TH>
+hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks);
 
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);  // zero fill kernel (graph safe)
 hipError_t launch_apply_arrest_f64(double* V, long long n, const DevStatus* st, hipStream_t s);

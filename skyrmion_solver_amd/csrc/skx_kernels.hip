@@ -1,590 +1,10 @@
-// Stage kernels of the arrested-Newton-flow step for gfx950.
-//
-// One kernel template covers the energy pass (stage 0) and the four RK4
-// gradient stages of one step.  Each stage is one pass over the lattice:
-//
-//   stage field   Temp_s = F + f_s * dt * (V + f'_s * l_{s-1})   (formed on load)
-//   stage 1       l1 = -dt g(Temp_1)            [+ energy / baryon of F fused]
-//   stage 2       l2 = -dt g(Temp_2)
-//   stage 3       l3 = -dt g(Temp_3)
-//   stage 4       l4 = -dt g(Temp_4); F', V' = RK4 combine; S^3 normalise and
-//                 tangent projection; signed-max and max|g| reductions fused
-//
-// which is the data flow of core/integrator.py:382-409 (k_s and Temp are
-// never stored; they are recomputed from F, V and the l_s on the fly).
-//
-// Tiling: lanes run along the unit-stride axis (reference z), a block covers
-// a TZ x TY tile of the (y, z) plane and marches along the slowest axis
-// (reference x) over a chunk of planes.  Each incoming plane of the stage
-// field (with a 2-site y/z halo) is staged in a 4-slot LDS ring, so the
-// centre plane (two behind the lead) is still resident for its y/z second
-// differences; one barrier per plane.  Per-site values along x (the field
-// and its raw y / z first differences) live in 5-deep register queues, and
-// raw z-differences of every tile row go to a 3-slot LDS ring for the yz
-// mixed derivative.  Mixed derivatives are first differences of raw first
-// differences, which is the reference's 25-tap tensor stencil
-// (core/derivatives.py:121-173) regrouped.
-#include <type_traits>
-
-#include "skx_internal.h"
+// Small kernels of the runtime (zero fill, arrest) and the dispatch from
+// runtime theory / precision / stage / tile shape to the stage kernels, which
+// are instantiated one (precision, theory) per translation unit
+// (skx_stage_<theory>_<precision>.hip).
+#include "skx_device.h"
 
 namespace skx {
-
-template <int TH>
-struct Traits;
-template <>
-struct Traits<PION> {
-  static constexpr int NF = 4;
-  static constexpr int NSK = 4;
-  static constexpr unsigned MIXED = 0xFu;  // components needing mixed second derivatives
-};
-template <>
-struct Traits<OMEGA> {
-  static constexpr int NF = 5;
-  static constexpr int NSK = 4;
-  static constexpr unsigned MIXED = 0x0u;
-};
-template <>
-struct Traits<COULOMB> {
-  static constexpr int NF = 5;
-  static constexpr int NSK = 4;
-  static constexpr unsigned MIXED = 0xFu;
-};
-
-// (1/12 f[-2] - 2/3 f[-1] + 2/3 f[+1] - 1/12 f[+2]) / ls   (core/derivatives.py:55)
-template <typename T>
-__device__ __forceinline__ T st_d1(T fm2, T fm1, T fp1, T fp2, T ils) {
-  return (T(1.0 / 12.0) * (fm2 - fp2) + T(2.0 / 3.0) * (fp1 - fm1)) * ils;
-}
-// (-1/12 f[-2] + 4/3 f[-1] - 5/2 f[0] + 4/3 f[+1] - 1/12 f[+2]) / ls^2   (core/derivatives.py:107)
-template <typename T>
-__device__ __forceinline__ T st_d2(T fm2, T fm1, T f0, T fp1, T fp2, T ils2) {
-  return (T(4.0 / 3.0) * (fm1 + fp1) - T(1.0 / 12.0) * (fm2 + fp2) - T(2.5) * f0) * ils2;
-}
-
-template <int TH, typename T, int LD>
-__device__ __forceinline__ void point_gradient(const T* phi, const T (*F)[LD], const T (*S)[LD], const PhysParams& P,
-                                               T* g) {
-  if constexpr (TH == PION) gradient_pion<T>(phi, F, S, P, g);
-  if constexpr (TH == OMEGA) gradient_omega<T>(phi, F, S, P, g);
-  if constexpr (TH == COULOMB) gradient_coulomb<T>(phi, F, S, P, g);
-}
-
-template <int TH, typename T, int LD>
-__device__ __forceinline__ double point_energy(const T* phi, const T (*F)[LD], const PhysParams& P) {
-  if constexpr (TH == PION) return energy_pion<T, LD>(phi, F, P);
-  if constexpr (TH == OMEGA) return energy_omega<T, LD>(phi, F, P);
-  if constexpr (TH == COULOMB) return energy_coulomb<T, LD>(phi, F, P);
-  return 0.0;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-__device__ __forceinline__ double load_partial(const double* p) {
-  unsigned long long u =
-      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double(static_cast<long long>(u));
-}
-
-// Two-value block reduction, then the last block to finish folds every
-// block's partials in a fixed order (deterministic, no memset needed).
-// IS_MAX selects max instead of (Neumaier-compensated) sum.  Returns true in
-// thread 0 of the last block, with the grid-wide results in r0 / r1.
-template <int NT, bool IS_MAX>
-__device__ bool grid_reduce2(double& r0, double& r1, double* partial, uint32_t* counter, double* red, int blk,
-                             int nblk) {
-  constexpr int NW = NT / 64;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  double a = IS_MAX ? wave_max(r0) : wave_sum(r0);
-  double b = IS_MAX ? wave_max(r1) : wave_sum(r1);
-  if (lane == 0) {
-    red[w] = a;
-    red[NW + w] = b;
-  }
-  __syncthreads();
-  __shared__ bool is_last;
-  if (tid == 0) {
-    double s0 = red[0], s1 = red[NW];
-    for (int i = 1; i < NW; ++i) {
-      s0 = IS_MAX ? fmax(s0, red[i]) : s0 + red[i];
-      s1 = IS_MAX ? fmax(s1, red[NW + i]) : s1 + red[NW + i];
-    }
-    partial[2 * blk] = s0;
-    partial[2 * blk + 1] = s1;
-    __threadfence();
-    const uint32_t prev = atomicAdd(counter, 1u);
-    is_last = (prev == (uint32_t)nblk - 1u);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  const int nb = nblk;
-  double s0 = IS_MAX ? -1.0e300 : 0.0, c0 = 0.0;
-  double s1 = IS_MAX ? -1.0e300 : 0.0, c1 = 0.0;
-  for (int i = tid; i < nb; i += NT) {
-    const double v0 = load_partial(partial + 2 * i);
-    const double v1 = load_partial(partial + 2 * i + 1);
-    if (IS_MAX) {
-      s0 = fmax(s0, v0);
-      s1 = fmax(s1, v1);
-    } else {  // Neumaier compensated summation
-      double t = s0 + v0;
-      c0 += (fabs(s0) >= fabs(v0)) ? ((s0 - t) + v0) : ((v0 - t) + s0);
-      s0 = t;
-      t = s1 + v1;
-      c1 += (fabs(s1) >= fabs(v1)) ? ((s1 - t) + v1) : ((v1 - t) + s1);
-      s1 = t;
-    }
-  }
-  if (!IS_MAX) {
-    s0 += c0;
-    s1 += c1;
-  }
-  __syncthreads();
-  a = IS_MAX ? wave_max(s0) : wave_sum(s0);
-  b = IS_MAX ? wave_max(s1) : wave_sum(s1);
-  if (lane == 0) {
-    red[w] = a;
-    red[NW + w] = b;
-  }
-  __syncthreads();
-  if (tid != 0) return false;
-  s0 = red[0];
-  s1 = red[NW];
-  for (int i = 1; i < NW; ++i) {
-    s0 = IS_MAX ? fmax(s0, red[i]) : s0 + red[i];
-    s1 = IS_MAX ? fmax(s1, red[NW + i]) : s1 + red[NW + i];
-  }
-  *counter = 0u;  // ready for the next launch
-  r0 = s0;
-  r1 = s1;
-  return true;
-}
-
-template <int K>
-using IC = std::integral_constant<int, K>;
-
-// Stencil coefficients of one axis with the inverse spacing folded in.
-template <typename T>
-struct AxisCoef {
-  T d1a, d1b;       // D1 = d1a (f[-2] - f[+2]) + d1b (f[+1] - f[-1])
-  T d2a, d2b, d2c;  // D2 = d2a (f[-1] + f[+1]) - d2b (f[-2] + f[+2]) - d2c f[0]
-  __device__ void set(T ils) {
-    d1a = T(1.0 / 12.0) * ils;
-    d1b = T(2.0 / 3.0) * ils;
-    const T i2 = ils * ils;
-    d2a = T(4.0 / 3.0) * i2;
-    d2b = T(1.0 / 12.0) * i2;
-    d2c = T(2.5) * i2;
-  }
-};
-
-template <typename T>
-__device__ __forceinline__ T cd1(T fm2, T fm1, T fp1, T fp2, T a, T b) {
-  return a * (fm2 - fp2) + b * (fp1 - fm1);
-}
-template <typename T>
-__device__ __forceinline__ T cd2(T fm2, T fm1, T f0, T fp1, T fp2, T a, T b, T c) {
-  return a * (fm1 + fp1) - b * (fm2 + fp2) - c * f0;
-}
-
-// Buffer addressing: one resource per (array, component, plane) with the
-// plane's byte size as the range, so every load / store takes a 32-bit lane
-// offset and the scalar base; offsets at or beyond the range (OOB) read 0 and
-// drop stores in hardware, which is how halo cells outside the lattice are
-// zeroed.
-constexpr uint32_t kOOB = 0x80000000u;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
-}
-template <typename T>
-__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-  if constexpr (sizeof(T) == 4) {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-  } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-  }
-}
-template <typename T>
-__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
-  if constexpr (sizeof(T) == 4) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, 0, 0);
-  } else {
-    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
-  }
-}
-
-// One stage over the block `blk` of `nblk` blocks that cover one lattice.
-template <typename T, int TH, int STAGE, int TZ, int TY>
-__device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int nblk) {
-  using Tr = Traits<TH>;
-  constexpr int NF = Tr::NF;
-  constexpr int NSK = Tr::NSK;
-  constexpr int NT = TZ * TY;
-  constexpr int TW = TZ + 4;  // tile row length (z with halo)
-  constexpr int TR = TY + 4;  // tile rows (y with halo)
-  constexpr int TE = TW * TR;
-  constexpr int LPT = (TE + NT - 1) / NT;
-  constexpr int NA = (STAGE <= 1) ? 1 : (STAGE == 2 ? 2 : 3);
-  constexpr bool D2 = STAGE >= 1;
-  constexpr bool ANY_MIXED = D2 && (Tr::MIXED != 0u);
-  constexpr int FSLOTS = 4;  // stage-field planes: lead p, p-1, centre p-2, and the slot being refilled
-  constexpr int ZSLOTS = 3;  // raw z-differences for the yz mixed derivative: planes p, p-1, p-2
-  static_assert(NT % 64 == 0, "block must be whole wavefronts");
-  static_assert(TY >= 4, "halo rows of the z-difference ring are filled by the first four rows of threads");
-
-  __shared__ T fring[FSLOTS][NF][TE];
-  __shared__ T zring[ANY_MIXED ? ZSLOTS : 1][ANY_MIXED ? NF : 1][ANY_MIXED ? TR * TZ : 1];
-  __shared__ double red[2 * (NT / 64)];
-
-  if ((A.flags & CHECK_DONE) && A.st->done) return;  // uniform per launch
-
-  const int tid = threadIdx.x;
-  const int tz = tid % TZ, ty = tid / TZ;
-  // XCD-aware order: consecutive tiles (which share y halos) on the same L2
-  int lin = blk;
-  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
-  const int bz = lin % A.nbz;
-  lin /= A.nbz;
-  const int by = lin % A.nby;
-  const int bx = lin / A.nby;
-  const int z0 = bz * TZ, y0 = by * TY;
-  const int xs = bx * A.xchunk;
-  const int xe = min(xs + A.xchunk, A.X);
-  const int y = y0 + ty, z = z0 + tz;
-  const int Y = A.Y, Z = A.Z, X = A.X, halo = A.halo;
-  const bool own = (y < Y) && (z < Z);
-  const size_t N = (size_t)A.N;
-  const size_t YZ = (size_t)Y * Z;
-
-  const bool gy1 = (y >= halo) && (y < Y - halo), gz1 = (z >= halo) && (z < Z - halo);
-  const bool gy2 = (y > halo) && (y < Y - halo), gz2 = (z > halo) && (z < Z - halo);
-
-  const T dt = T(A.P.dt);
-  T keep = T(1);
-  if constexpr (STAGE >= 2) keep = T(A.st->keep);
-  AxisCoef<T> cx, cy, cz;
-  cx.set(A.ilsx);
-  cy.set(A.ilsy);
-  cz.set(A.ilsz);
-  // per-thread guarded coefficients (the guards of core/derivatives.py:54,106,121 along y and z)
-  const T y2a = gy1 ? cy.d2a : T(0), y2b = gy1 ? cy.d2b : T(0), y2c = gy1 ? cy.d2c : T(0);
-  const T z2a = gz1 ? cz.d2a : T(0), z2b = gz1 ? cz.d2b : T(0), z2c = gz1 ? cz.d2c : T(0);
-  const T mxy_a = gy2 ? cx.d1a : T(0), mxy_b = gy2 ? cx.d1b : T(0);  // x-part of d_x d_y, y guard folded
-  const T mxz_a = gz2 ? cx.d1a : T(0), mxz_b = gz2 ? cx.d1b : T(0);
-  const T myz_a = (gy2 && gz2) ? cy.d1a : T(0), myz_b = (gy2 && gz2) ? cy.d1b : T(0);
-
-  // tile elements this thread stages: byte offsets within a plane, kOOB for
-  // cells outside the lattice (they load as 0)
-  uint32_t eoff[LPT];
-#pragma unroll
-  for (int j = 0; j < LPT; ++j) {
-    const int e = tid + j * NT;
-    const int r = e / TW, cc = e - r * TW;
-    const int gy = y0 - 2 + r, gz = z0 - 2 + cc;
-    const bool valid = (e < TE) && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
-    eoff[j] = valid ? (uint32_t)(gy * Z + gz) * (uint32_t)sizeof(T) : kOOB;
-  }
-  const uint32_t own_off = own ? (uint32_t)(y * Z + z) : 0u;
-  const uint32_t own_b = own ? own_off * (uint32_t)sizeof(T) : kOOB;
-  const uint32_t plane_bytes = (uint32_t)(YZ * sizeof(T));
-  // halo row of the z-difference ring this thread fills (rows 0,1 and TY+2,TY+3)
-  const int hrow = (ty < 2) ? ty : (TY + ty);
-
-  const T* src[3] = {A.F, A.V, A.la};
-  T pre[NA][NF][LPT];
-  auto issue = [&](int p) {
-    if (p < 0 || p >= X) {  // uniform: planes outside the lattice are zero
-#pragma unroll
-      for (int k = 0; k < NA; ++k)
-#pragma unroll
-        for (int a = 0; a < NF; ++a)
-#pragma unroll
-          for (int j = 0; j < LPT; ++j) pre[k][a][j] = T(0);
-      return;
-    }
-#pragma unroll
-    for (int k = 0; k < NA; ++k)
-#pragma unroll
-      for (int a = 0; a < NF; ++a) {
-        const auto r = plane_rsrc(src[k] + a * N + (size_t)p * YZ, plane_bytes);
-#pragma unroll
-        for (int j = 0; j < LPT; ++j) pre[k][a][j] = bload<T>(r, eoff[j]);
-      }
-  };
-
-  // own-column x-queues, 5 slots rotated by the unrolled plane loop
-  T qf[NF][5], qy[NF][5], qz[NF][5];
-#pragma unroll
-  for (int a = 0; a < NF; ++a)
-#pragma unroll
-    for (int k = 0; k < 5; ++k) qf[a][k] = qy[a][k] = qz[a][k] = T(0);
-
-  double acc0 = (STAGE == 4) ? -1.0e300 : 0.0;
-  double acc1 = 0.0;
-  bool bad = false;
-  const int C = (ty + 2) * TW + (tz + 2);
-
-  // One plane of the march.  K = phase mod 5: the lead plane p goes to queue
-  // slot K; the centre c = p - 2 and its x-neighbours c-2..c+2 sit in slots
-  // (K+1)..(K+5) mod 5.
-  auto body = [&](auto Kc, int p) {
-    constexpr int K = decltype(Kc)::value;
-    constexpr int sm2 = (K + 1) % 5, sm1 = (K + 2) % 5, s0 = (K + 3) % 5, sp1 = (K + 4) % 5, sp2 = K;
-    const int fs = (p + 8) & (FSLOTS - 1);  // p >= -2
-#pragma unroll
-    for (int a = 0; a < NF; ++a)
-#pragma unroll
-      for (int j = 0; j < LPT; ++j) {
-        const int e = tid + j * NT;
-        if (j < LPT - 1 || e < TE) {
-          T v = pre[0][a][j];
-          if constexpr (STAGE == 2) v = v + T(0.5) * (dt * (keep != T(0) ? pre[1][a][j] : T(0)));
-          if constexpr (STAGE == 3)
-            v = v + T(0.5) * (dt * ((keep != T(0) ? pre[1][a][j] : T(0)) + T(0.5) * pre[2][a][j]));
-          if constexpr (STAGE == 4) v = v + dt * ((keep != T(0) ? pre[1][a][j] : T(0)) + T(0.5) * pre[2][a][j]);
-          fring[fs][a][e] = v;
-        }
-      }
-    __syncthreads();
-    if (p + 1 <= xe + 1) issue(p + 1);
-
-    const int c = p - 2;
-    const bool center = (c >= xs) && own;
-    // stage 4 reads its own site's F, V, l1, l2, l3 for the RK4 combine: issue early
-    T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
-    if constexpr (STAGE == 4) {
-      if (c >= xs) {
-        const size_t pb = (size_t)c * YZ;
-#pragma unroll
-        for (int a = 0; a < NF; ++a) {
-          const size_t o = a * N + pb;
-          cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), own_b);
-          cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), own_b);
-          cl1[a] = bload<T>(plane_rsrc(A.lb + o, plane_bytes), own_b);
-          cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), own_b);
-          cl3[a] = bload<T>(plane_rsrc(A.lc + o, plane_bytes), own_b);
-        }
-      }
-    }
-
-    // lead plane p: own-column value and raw first differences along y and z
-    const int zs = (p + 9) % ZSLOTS;
-#pragma unroll
-    for (int a = 0; a < NF; ++a) {
-      const T* t = fring[fs][a];
-      qf[a][sp2] = t[C];
-      qy[a][sp2] = cd1(t[C - 2 * TW], t[C - TW], t[C + TW], t[C + 2 * TW], cy.d1a, cy.d1b);
-      const T dz = cd1(t[C - 2], t[C - 1], t[C + 1], t[C + 2], cz.d1a, cz.d1b);
-      qz[a][sp2] = dz;
-      if constexpr (ANY_MIXED) {
-        if ((Tr::MIXED >> a) & 1u) {
-          zring[zs][a][(ty + 2) * TZ + tz] = dz;
-          if (ty < 4) {
-            const int h = hrow * TW + tz + 2;
-            zring[zs][a][hrow * TZ + tz] = cd1(t[h - 2], t[h - 1], t[h + 1], t[h + 2], cz.d1a, cz.d1b);
-          }
-        }
-      }
-    }
-
-    if (center) {
-      // x guards are uniform over the block for this plane
-      const bool gx1 = (c >= halo) && (c < X - halo), gx2 = (c > halo) && (c < X - halo);
-      const T x1a = gx1 ? cx.d1a : T(0), x1b = gx1 ? cx.d1b : T(0);
-      const T x2a = gx1 ? cx.d2a : T(0), x2b = gx1 ? cx.d2b : T(0), x2c = gx1 ? cx.d2c : T(0);
-      const T xya = gx2 ? mxy_a : T(0), xyb = gx2 ? mxy_b : T(0);
-      const T xza = gx2 ? mxz_a : T(0), xzb = gx2 ? mxz_b : T(0);
-      const int cfs = (c + 8) & (FSLOTS - 1);
-      const int czs = (c + 9) % ZSLOTS;
-      T phi[NF], F[3][NF], S[6][NF];
-#pragma unroll
-      for (int a = 0; a < NF; ++a) {
-        const T f0 = qf[a][s0];
-        phi[a] = f0;
-        F[0][a] = cd1(qf[a][sm2], qf[a][sm1], qf[a][sp1], qf[a][sp2], x1a, x1b);
-        F[1][a] = gy1 ? qy[a][s0] : T(0);
-        F[2][a] = gz1 ? qz[a][s0] : T(0);
-        if constexpr (D2) {
-          const T* t = fring[cfs][a];
-          S[0][a] = cd2(qf[a][sm2], qf[a][sm1], f0, qf[a][sp1], qf[a][sp2], x2a, x2b, x2c);
-          S[1][a] = cd2(t[C - 2 * TW], t[C - TW], f0, t[C + TW], t[C + 2 * TW], y2a, y2b, y2c);
-          S[2][a] = cd2(t[C - 2], t[C - 1], f0, t[C + 1], t[C + 2], z2a, z2b, z2c);
-          if ((Tr::MIXED >> a) & 1u) {
-            S[3][a] = cd1(qy[a][sm2], qy[a][sm1], qy[a][sp1], qy[a][sp2], xya, xyb);
-            S[4][a] = cd1(qz[a][sm2], qz[a][sm1], qz[a][sp1], qz[a][sp2], xza, xzb);
-            if constexpr (ANY_MIXED) {
-              const T* zr = zring[czs][a];
-              S[5][a] = cd1(zr[ty * TZ + tz], zr[(ty + 1) * TZ + tz], zr[(ty + 3) * TZ + tz], zr[(ty + 4) * TZ + tz],
-                            myz_a, myz_b);
-            }
-          } else {
-            S[3][a] = S[4][a] = S[5][a] = T(0);
-          }
-        }
-      }
-      const size_t so = (size_t)c * YZ + own_off;
-
-      if constexpr (STAGE <= 1) {
-        if (STAGE == 0 || (A.flags & FUSE_ENERGY)) {
-          const double e = point_energy<TH, T, NF>(phi, F, A.P);
-          acc0 += e;
-          if (STAGE == 0 && A.dens_e) A.dens_e[so] = e;
-        }
-        if (A.flags & FUSE_BARYON) {
-          const double b = baryon_density<T, NF>(phi, F, A.P);
-          acc1 += b;
-          if (STAGE == 0 && A.dens_b) A.dens_b[so] = b;
-        }
-      }
-      if constexpr (D2) {
-        T g[NF];
-        point_gradient<TH, T, NF>(phi, F, S, A.P, g);
-        if (A.eg_out) {
-#pragma unroll
-          for (int a = 0; a < NF; ++a) A.eg_out[a * N + so] = double(g[a]);
-        }
-        if constexpr (STAGE <= 3) {
-#pragma unroll
-          for (int a = 0; a < NF; ++a) bstore<T>(-dt * g[a], plane_rsrc(A.lout + a * N + (size_t)c * YZ, plane_bytes), own_b);
-        } else {
-          // RK4 combine (core/integrator.py:170-175), recomputing k1..k4 from V and l1..l3
-          using R = double;
-          R Fn[NF], Vn[NF];
-          const R rdt = R(A.P.dt);
-#pragma unroll
-          for (int a = 0; a < NF; ++a) {
-            const R Fv = R(cF[a]);
-            const R Vk = (keep != T(0)) ? R(cV[a]) : R(0);
-            const R l1 = R(cl1[a]), l2 = R(cl2[a]), l3 = R(cl3[a]);
-            const R l4 = R(-dt * g[a]);
-            const R k1 = rdt * Vk;
-            const R k2 = rdt * (Vk + 0.5 * l1);
-            const R k3 = rdt * (Vk + 0.5 * l2);
-            const R k4 = rdt * (Vk + l3);
-            Vn[a] = Vk + (1.0 / 6.0) * (l1 + 2.0 * l2 + 2.0 * l3 + l4);
-            Fn[a] = Fv + (1.0 / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
-            const double gv = double(g[a]);
-            acc0 = fmax(acc0, gv);
-            acc1 = fmax(acc1, fabs(gv));
-            bad |= !isfinite(gv);
-          }
-          if (A.unit_length) {
-            R s = 0.0;
-#pragma unroll
-            for (int a = 0; a < NSK; ++a) s += Fn[a] * Fn[a];
-            s = sqrt(s);
-            if (s != 0.0) {
-#pragma unroll
-              for (int a = 0; a < NSK; ++a) Fn[a] /= s;
-            }
-            R lm = 0.0;
-#pragma unroll
-            for (int a = 0; a < NSK; ++a) lm += Vn[a] * Fn[a];
-#pragma unroll
-            for (int a = 0; a < NSK; ++a) Vn[a] -= lm * Fn[a];
-          }
-#pragma unroll
-          for (int a = 0; a < NF; ++a) {
-            bstore<T>(T(Fn[a]), plane_rsrc(A.Fout + a * N + (size_t)c * YZ, plane_bytes), own_b);
-            bstore<T>(T(Vn[a]), plane_rsrc(A.Vout + a * N + (size_t)c * YZ, plane_bytes), own_b);
-          }
-        }
-      }
-    }
-  };
-
-  issue(xs - 2);
-  const int pend = xe + 1;
-  for (int p = xs - 2;;) {
-    body(IC<0>{}, p);
-    if (++p > pend) break;
-    body(IC<1>{}, p);
-    if (++p > pend) break;
-    body(IC<2>{}, p);
-    if (++p > pend) break;
-    body(IC<3>{}, p);
-    if (++p > pend) break;
-    body(IC<4>{}, p);
-    if (++p > pend) break;
-  }
-
-  // ---- fused reductions
-  if constexpr (STAGE <= 1) {
-    if (STAGE == 1 && !(A.flags & (FUSE_ENERGY | FUSE_BARYON))) return;
-    double r0 = acc0, r1 = acc1;
-    if (grid_reduce2<NT, false>(r0, r1, A.partial, &A.st->cnt_sum, red, blk, nblk)) {
-      DevStatus* st = A.st;
-      if (A.flags & FUSE_ENERGY) st->energy = r0;
-      if (A.flags & FUSE_BARYON) st->baryon = r1;
-      if (!isfinite(r0) || !isfinite(r1)) {
-        st->nonfinite = 1;
-        st->done = 1;
-      }
-      if (A.flags & DECIDE_ARREST) {
-        double kp = 1.0;
-        if (st->arrest_pending && st->killkinen && (r0 > st->e_prev)) kp = 0.0;
-        st->keep = kp;
-        st->arrest_pending = 0;
-        st->e_prev = r0;
-      }
-      if (A.hist && (A.flags & DECIDE_ARREST) && st->steps < st->hist_cap) {
-        A.hist[st->steps].energy = r0;
-        A.hist[st->steps].baryon = r1;
-      }
-    }
-  } else if constexpr (STAGE == 4) {
-    // +inf in the max|g| slot marks a non-finite gradient
-    double r0 = acc0, r1 = bad ? __longlong_as_double(0x7ff0000000000000LL) : acc1;
-    if (grid_reduce2<NT, true>(r0, r1, A.partial, &A.st->cnt_max, red, blk, nblk)) {
-      DevStatus* st = A.st;
-      st->err_signed = r0;
-      st->err_abs = r1;
-      if (!isfinite(r1)) {
-        st->nonfinite = 1;
-        st->done = 1;
-      }
-      if (A.flags & RUN_MODE) {
-        if (A.hist && st->steps < st->hist_cap) {
-          A.hist[st->steps].err_signed = r0;
-          A.hist[st->steps].err_abs = r1;
-        }
-        st->steps += 1;
-        st->arrest_pending = 1;
-        if (r1 < st->tol || st->steps >= st->max_steps) st->done = 1;
-      }
-    }
-  }
-}
-
-template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY) k_stage(const StageArgs<T> A) {
-  stage_body<T, TH, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
-}
-
-// Batched launch: many lattices of one theory / precision / tile shape in one
-// grid.  blk_inst maps every block of the grid to a slot (-1 = idle block);
-// the slot carries the lattice's arguments for both ping-pong parities.
-template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY) k_stage_batch(const BatchSlot<T>* __restrict__ slots,
-                                                       const int* __restrict__ blk_inst, int parity) {
-  const int inst = blk_inst[blockIdx.x];
-  if (inst < 0) return;
-  const BatchSlot<T>& sl = slots[inst];
-  if (!sl.active) return;
-  stage_body<T, TH, STAGE, TZ, TY>(sl.args[parity], (int)blockIdx.x - sl.first_block, sl.nblocks[STAGE]);
-}
 
 // ------------------------------------------------------------------ small kernels
 __global__ void k_fill_zero(uint32_t* p, size_t n) {
@@ -630,96 +50,52 @@ bool shape_supported(const LaunchShape& s) {
   return (s.tz == 64 && s.ty == 4) || (s.tz == 32 && s.ty == 8) || (s.tz == 16 && s.ty == 16);
 }
 
-// Visit the kernel instantiation for (T, theory, stage, shape) with a functor
-// taking the kernel's function pointer type as a template argument.
-template <typename T, int TH, int TZ, int TY, class Fn>
-static hipError_t visit_stage(int stage, Fn&& fn) {
-  switch (stage) {
-    case 0: return fn.template operator()<T, TH, 0, TZ, TY>();
-    case 1: return fn.template operator()<T, TH, 1, TZ, TY>();
-    case 2: return fn.template operator()<T, TH, 2, TZ, TY>();
-    case 3: return fn.template operator()<T, TH, 3, TZ, TY>();
-    case 4: return fn.template operator()<T, TH, 4, TZ, TY>();
-  }
-  return hipErrorInvalidValue;
-}
-template <typename T, int TZ, int TY, class Fn>
-static hipError_t visit_theory(int theory, int stage, Fn&& fn) {
-  switch (theory) {
-    case PION: return visit_stage<T, PION, TZ, TY>(stage, fn);
-    case OMEGA: return visit_stage<T, OMEGA, TZ, TY>(stage, fn);
-    case COULOMB: return visit_stage<T, COULOMB, TZ, TY>(stage, fn);
-  }
-  return hipErrorInvalidValue;
-}
-template <typename T, class Fn>
-static hipError_t visit(int theory, int stage, const LaunchShape& s, Fn&& fn) {
-  if (s.tz == 64 && s.ty == 4) return visit_theory<T, 64, 4>(theory, stage, fn);
-  if (s.tz == 32 && s.ty == 8) return visit_theory<T, 32, 8>(theory, stage, fn);
-  if (s.tz == 16 && s.ty == 16) return visit_theory<T, 16, 16>(theory, stage, fn);
-  return hipErrorInvalidValue;
-}
-
-template <typename TA>
-struct Launcher {
-  const StageArgs<TA>& a;
-  const LaunchShape& s;
-  hipStream_t st;
-  template <typename T, int TH, int STAGE, int TZ, int TY>
-  hipError_t operator()() {
-    StageArgs<T> b = a;
-    b.xchunk = s.xchunk;
-    b.nbz = (a.Z + TZ - 1) / TZ;
-    b.nby = (a.Y + TY - 1) / TY;
-    b.nbx = (a.X + s.xchunk - 1) / s.xchunk;
-    const int blocks = b.nbz * b.nby * b.nbx;
-    hipLaunchKernelGGL((k_stage<T, TH, STAGE, TZ, TY>), dim3(blocks), dim3(TZ * TY), 0, st, b);
-    return hipGetLastError();
-  }
-};
-
-struct Occupancy {
-  int* out;
-  template <typename T, int TH, int STAGE, int TZ, int TY>
-  hipError_t operator()() {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, reinterpret_cast<const void*>(&k_stage<T, TH, STAGE, TZ, TY>),
-                                                        TZ * TY, 0);
-  }
-};
-
 hipError_t launch_stage_f64(int theory, int stage, const StageArgs<double>& a, const LaunchShape& s, hipStream_t st) {
-  return visit<double>(theory, stage, s, Launcher<double>{a, s, st});
+  switch (theory) {
+    case PION: return stage_launch<double, PION>(stage, a, s, st);
+    case OMEGA: return stage_launch<double, OMEGA>(stage, a, s, st);
+    case COULOMB: return stage_launch<double, COULOMB>(stage, a, s, st);
+    case RHO: return stage_launch<double, RHO>(stage, a, s, st);
+  }
+  return hipErrorInvalidValue;
 }
 hipError_t launch_stage_f32(int theory, int stage, const StageArgs<float>& a, const LaunchShape& s, hipStream_t st) {
-  return visit<float>(theory, stage, s, Launcher<float>{a, s, st});
-}
-
-template <typename TA>
-struct BatchLauncher {
-  const BatchSlot<TA>* slots;
-  const int* blk_inst;
-  int parity, grid;
-  hipStream_t st;
-  template <typename T, int TH, int STAGE, int TZ, int TY>
-  hipError_t operator()() {
-    hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY>), dim3(grid), dim3(TZ * TY), 0, st,
-                       reinterpret_cast<const BatchSlot<T>*>(slots), blk_inst, parity);
-    return hipGetLastError();
+  switch (theory) {
+    case PION: return stage_launch<float, PION>(stage, a, s, st);
+    case OMEGA: return stage_launch<float, OMEGA>(stage, a, s, st);
+    case COULOMB: return stage_launch<float, COULOMB>(stage, a, s, st);
+    case RHO: return stage_launch<float, RHO>(stage, a, s, st);
   }
-};
+  return hipErrorInvalidValue;
+}
 
 hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, const BatchSlot<double>* slots,
                                   const int* blk_inst, int parity, int grid, hipStream_t st) {
-  return visit<double>(theory, stage, s, BatchLauncher<double>{slots, blk_inst, parity, grid, st});
+  switch (theory) {
+    case PION: return stage_launch_batch<double, PION>(stage, s, slots, blk_inst, parity, grid, st);
+    case OMEGA: return stage_launch_batch<double, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
+    case COULOMB: return stage_launch_batch<double, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
+  }
+  return hipErrorInvalidValue;
 }
 hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, const BatchSlot<float>* slots,
                                   const int* blk_inst, int parity, int grid, hipStream_t st) {
-  return visit<float>(theory, stage, s, BatchLauncher<float>{slots, blk_inst, parity, grid, st});
+  switch (theory) {
+    case PION: return stage_launch_batch<float, PION>(stage, s, slots, blk_inst, parity, grid, st);
+    case OMEGA: return stage_launch_batch<float, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
+    case COULOMB: return stage_launch_batch<float, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t stage_blocks_per_cu(int prec_fp32, int theory, int stage, const LaunchShape& s, int* blocks) {
-  Occupancy o{blocks};
-  return prec_fp32 ? visit<float>(theory, stage, s, o) : visit<double>(theory, stage, s, o);
+  switch (theory) {
+    case PION: return prec_fp32 ? stage_occupancy<float, PION>(stage, s, blocks) : stage_occupancy<double, PION>(stage, s, blocks);
+    case OMEGA: return prec_fp32 ? stage_occupancy<float, OMEGA>(stage, s, blocks) : stage_occupancy<double, OMEGA>(stage, s, blocks);
+    case COULOMB: return prec_fp32 ? stage_occupancy<float, COULOMB>(stage, s, blocks) : stage_occupancy<double, COULOMB>(stage, s, blocks);
+    case RHO: return prec_fp32 ? stage_occupancy<float, RHO>(stage, s, blocks) : stage_occupancy<double, RHO>(stage, s, blocks);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace skx

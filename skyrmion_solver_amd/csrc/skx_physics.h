@@ -290,6 +290,71 @@ SKX_HD void gradient_rho(const T* phi, const T (*F)[4], const T (*S)[13], const 
   project_tangent<T, 4>(g, phi);
 }
 
+// The rho gradient of theories/rho_skyrme/kernels.py:607-682 with the rho
+// second derivatives entering through
+//   D[a][j] = sum_i d_i R^a_ij = sum_{i != j} (S_ii,rho(j,a) - S_ij,rho(i,a)),
+// which is how they appear there: the back-reaction sums _d_R_a_ijk(S, a, i, j, i)
+// over i (kernels.py:655-664) and the rho equation sums
+// _d_R_a_ijk(S, a, i, j, j) = -D[a][i] over j (kernels.py:667-674).  F and S
+// hold the 4 pion components only.
+template <typename T>
+SKX_HD void gradient_rho_split(const T* phi, const T (*F)[4], const T (*S)[4], const T (*D)[3], const PhysParams& P,
+                               T g[13]) {
+  const T mpi2 = T(P.m2), mrho2 = T(P.coup), ca = T(P.calpha);
+  T nn[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) nn[j] = F[j][0] * F[j][0] + F[j][1] * F[j][1] + F[j][2] * F[j][2] + F[j][3] * F[j][3];
+  const T ntot = nn[0] + nn[1] + nn[2];
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    T acc = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const T Sii = S[i][mu];
+      acc += T(2) * Sii;
+      T t = Sii * ntot;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const T* Sij = S[sidx(i, j)];
+        const T* Sjj = S[j];
+        const T a1 = Sij[0] * F[j][0] + Sij[1] * F[j][1] + Sij[2] * F[j][2] + Sij[3] * F[j][3];
+        const T a2 = Sjj[0] * F[i][0] + Sjj[1] * F[i][1] + Sjj[2] * F[i][2] + Sjj[3] * F[i][3];
+        const T dij = F[i][0] * F[j][0] + F[i][1] * F[j][1] + F[i][2] * F[j][2] + F[i][3] * F[j][3];
+        t += F[i][mu] * (a1 - a2) - Sij[mu] * dij;
+      }
+      acc += T(2) * t;
+    }
+    g[mu] = -acc;
+  }
+  g[0] -= T(2) * mpi2;
+  const T c16 = T(16) * ca;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const T d = D[a][j];
+      g[0] += c16 * F[j][a + 1] * d;
+      g[a + 1] -= c16 * F[j][0] * d;
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double e = eps3(b, a, c);
+          if (e != 0.0) g[a + 1] += c16 * T(e) * F[j][c + 1] * D[b][j];
+        }
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      T om = T(0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) om += rho_dOmega<T, 4>(F, S, a, i, j, j);
+      g[rho_c(i, a)] = T(8) * mrho2 * phi[rho_c(i, a)] - T(8) * D[a][i] - c16 * om;
+    }
+  project_tangent<T, 4>(g, phi);
+}
+
 // ---------------------------------------------------------------- energy densities
 // All return e * vol in double; derivatives are promoted to double first.
 
@@ -379,6 +444,47 @@ SKX_HD double energy_rho(const T* phi, const T (*F)[13], const PhysParams& P) {
         e -= 8.0 * ca * R * Om;
       }
     }
+  return e * P.vol;
+}
+
+// theories/rho_skyrme/kernels.py:302-347 with the rho first derivatives
+// entering through R[a][p] = R^a_ij for the pairs p = (x,y), (x,z), (y,z):
+// R and Omega are antisymmetric in (i, j), so the ordered-pair sum of the
+// reference is twice the sum over these pairs.  F holds the pion components.
+template <typename T>
+SKX_HD double energy_rho_split(const T* phi, const T (*F)[4], const T (*R)[3], const PhysParams& P) {
+  const double mrho2 = P.coup, ca = P.calpha;
+  double f[3][4];
+  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < 4; ++a) f[i][a] = double(F[i][a]);
+  double n[3], d[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) d[i][j] = f[i][0] * f[j][0] + f[i][1] * f[j][1] + f[i][2] * f[j][2] + f[i][3] * f[j][3];
+  for (int i = 0; i < 3; ++i) n[i] = d[i][i];
+  double e = n[0] + n[1] + n[2];
+  double q = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) q += n[i] * n[j] - d[i][j] * d[i][j];
+  e += 0.5 * q;
+  e += 2.0 * P.m2 * (1.0 - double(phi[0]));
+  constexpr int PI[3] = {0, 0, 1}, PJ[3] = {1, 2, 2};
+  for (int a = 0; a < 3; ++a) {
+    for (int i = 0; i < 3; ++i) {
+      const double r = double(phi[rho_c(i, a)]);
+      e += 4.0 * mrho2 * r * r;
+    }
+    for (int p = 0; p < 3; ++p) {
+      const int i = PI[p], j = PJ[p];
+      const double Rij = double(R[a][p]);
+      double Om = f[i][0] * f[j][a + 1] - f[i][a + 1] * f[j][0];
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          const double ee = eps3(a, b, c);
+          if (ee != 0.0) Om += ee * f[i][b + 1] * f[j][c + 1];
+        }
+      e += 2.0 * (2.0 * Rij * Rij - 8.0 * ca * Rij * Om);
+    }
+  }
   return e * P.vol;
 }
 

@@ -95,7 +95,7 @@ int parse_desc(const skx_desc* d, Lattice& L) {
       L.P.m2 = pf[8] * pf[8];
       L.P.coup = pf[9] * pf[9];
       L.P.calpha = pf[10];
-      return fail(SKX_ENOTSUP, "rho theory kernels are not built in this version");
+      break;
     default:
       return fail(SKX_EINVAL, "unknown theory id");
   }
@@ -128,12 +128,15 @@ size_t elem_size(int prec) { return prec == SKX_FP64 ? sizeof(double) : sizeof(f
 struct skx_engine {
   Lattice L;
   int device = 0;
+  bool owns_stream = true;  // false for lattices of a batch (they share its stream)
+  bool owns_status = true;  // false for lattices of a batch (status lives in the batch's array)
   uint32_t flags = 0;
   LaunchShape shape[5]{};  // per stage kind: 0 energy pass, 1..4 RK4 stages
   hipStream_t stream = nullptr;
   void* F[2] = {nullptr, nullptr};
   void* V[2] = {nullptr, nullptr};
   void* l[3] = {nullptr, nullptr, nullptr};
+  void* aux = nullptr;  // rho: per-site D^a_j and R^a_ij of the curl pass (18 components)
   int cur = 0;
   DevStatus* st = nullptr;
   DevStatus* st_host = nullptr;  // pinned mirror
@@ -166,9 +169,13 @@ size_t max_blocks(const Lattice& L, const LaunchShape& tile) {
   return (size_t)stage_blocks(L.X, L.Y, L.Z, s);
 }
 
+size_t aux_bytes(const Lattice& L) {
+  return L.theory == SKX_THEORY_RHO ? (size_t)L.N * 18 * elem_size(L.prec) : 0;
+}
+
 size_t footprint(const Lattice& L, const LaunchShape& tile) {
   const size_t arr = (size_t)L.N * L.nf * elem_size(L.prec);
-  return 7 * arr + 2 * max_blocks(L, tile) * sizeof(double) + sizeof(DevStatus);
+  return 7 * arr + aux_bytes(L) + 2 * max_blocks(L, tile) * sizeof(double) + sizeof(DevStatus);
 }
 
 template <typename T>
@@ -183,6 +190,7 @@ StageArgs<T> make_args(skx_engine* e, int stage, int flags, int c, const void* f
   a.lout = nullptr;
   a.Fout = nullptr;
   a.Vout = nullptr;
+  a.aux = static_cast<T*>(e->aux);
   if (stage == 1) a.lout = static_cast<T*>(e->l[0]);
   if (stage == 2) a.lout = static_cast<T*>(e->l[1]);
   if (stage == 3) {
@@ -430,15 +438,15 @@ void release(skx_engine* e) {
   drop_graphs(e);
   resolve_trace(e);
   for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
-  for (void* p : {e->F[0], e->F[1], e->V[0], e->V[1], e->l[0], e->l[1], e->l[2]})
+  for (void* p : {e->F[0], e->F[1], e->V[0], e->V[1], e->l[0], e->l[1], e->l[2], e->aux})
     if (p) (void)hipFree(p);
-  if (e->st) (void)hipFree(e->st);
-  if (e->st_host) (void)hipHostFree(e->st_host);
+  if (e->st && e->owns_status) (void)hipFree(e->st);
+  if (e->st_host && e->owns_status) (void)hipHostFree(e->st_host);
   if (e->partial) (void)hipFree(e->partial);
   if (e->eg) (void)hipFree(e->eg);
   if (e->dens2) (void)hipFree(e->dens2);
   if (e->hist) (void)hipFree(e->hist);
-  if (e->stream) (void)hipStreamDestroy(e->stream);
+  if (e->stream && e->owns_stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
 
@@ -448,24 +456,11 @@ void release(skx_engine* e) {
     if (hipSetDevice((h)->device) != hipSuccess) return fail(SKX_EHIP, "hipSetDevice failed"); \
   } while (0)
 
-}  // namespace
-
-extern "C" {
-
-int skx_version(void) { return SKX_VERSION; }
-
-const char* skx_last_error(void) { return g_last_error.c_str(); }
-
-int skx_footprint_bytes(const skx_desc* desc, size_t* bytes) {
-  Lattice L;
-  if (int rc = parse_desc(desc, L)) return rc;
-  if (!bytes) return fail(SKX_EINVAL, "null output");
-  *bytes = footprint(L, choose_tile(L, desc));
-  return SKX_OK;
-}
-
-int skx_create(const skx_desc* desc, skx_handle* out) {
-  if (!out) return fail(SKX_EINVAL, "null output handle");
+// Build a handle.  `shared` / `ext_st` / `ext_st_host` are non-null for a
+// lattice of a batch (shared stream, status in the batch's arrays);
+// `xchunk_all` > 0 forces one x-chunk for every stage.
+int create_engine(const skx_desc* desc, hipStream_t shared, DevStatus* ext_st, DevStatus* ext_st_host,
+                  int xchunk_all, skx_engine** out) {
   *out = nullptr;
   Lattice L;
   if (int rc = parse_desc(desc, L)) return rc;
@@ -497,12 +492,20 @@ int skx_create(const skx_desc* desc, skx_handle* out) {
       return fail(SKX_EHIP, std::string("occupancy query: ") + hipGetErrorString(r));
     }
     e->shape[stg] = tile;
-    e->shape[stg].xchunk = choose_xchunk(L, tile, bpc, n_cu, desc->x_chunk);
+    e->shape[stg].xchunk = choose_xchunk(L, tile, bpc, n_cu, xchunk_all > 0 ? xchunk_all : desc->x_chunk);
   }
-  if ((r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
+  if (shared) {
+    e->stream = shared;
+    e->owns_stream = false;
+  } else if ((r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) {
     e->stream = nullptr;
     release(e);
     return fail(SKX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(r));
+  }
+  if (ext_st) {
+    e->st = ext_st;
+    e->st_host = ext_st_host;
+    e->owns_status = false;
   }
   const size_t arr = (size_t)L.N * L.nf * elem_size(L.prec);
   void** bufs[7] = {&e->F[0], &e->F[1], &e->V[0], &e->V[1], &e->l[0], &e->l[1], &e->l[2]};
@@ -513,16 +516,22 @@ int skx_create(const skx_desc* desc, skx_handle* out) {
       return fail(SKX_ENOMEM, "device allocation of lattice arrays failed");
     }
   }
+  if (aux_bytes(L) && hipMalloc(&e->aux, aux_bytes(L)) != hipSuccess) {
+    e->aux = nullptr;
+    release(e);
+    return fail(SKX_ENOMEM, "device allocation of the rho curl buffer failed");
+  }
   size_t nblk = 0;
   for (int stg = 0; stg < 5; ++stg) nblk = std::max(nblk, (size_t)stage_blocks(L.X, L.Y, L.Z, e->shape[stg]));
   if (hipMalloc(&e->partial, 2 * nblk * sizeof(double)) != hipSuccess ||
-      hipMalloc(&e->st, sizeof(DevStatus)) != hipSuccess ||
-      hipHostMalloc(&e->st_host, sizeof(DevStatus), hipHostMallocDefault) != hipSuccess) {
+      (!ext_st && hipMalloc(&e->st, sizeof(DevStatus)) != hipSuccess) ||
+      (!ext_st && hipHostMalloc(&e->st_host, sizeof(DevStatus), hipHostMallocDefault) != hipSuccess)) {
     release(e);
     return fail(SKX_ENOMEM, "device allocation of reduction state failed");
   }
   // zero everything with a fill kernel (no memset nodes anywhere)
   for (void** b : bufs) (void)launch_fill(*b, arr, e->stream);
+  if (e->aux) (void)launch_fill(e->aux, aux_bytes(L), e->stream);
   (void)launch_fill(e->partial, 2 * nblk * sizeof(double), e->stream);
   std::memset(e->st_host, 0, sizeof(DevStatus));
   e->st_host->keep = 1.0;
@@ -536,6 +545,27 @@ int skx_create(const skx_desc* desc, skx_handle* out) {
   }
   *out = e;
   return SKX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int skx_version(void) { return SKX_VERSION; }
+
+const char* skx_last_error(void) { return g_last_error.c_str(); }
+
+int skx_footprint_bytes(const skx_desc* desc, size_t* bytes) {
+  Lattice L;
+  if (int rc = parse_desc(desc, L)) return rc;
+  if (!bytes) return fail(SKX_EINVAL, "null output");
+  *bytes = footprint(L, choose_tile(L, desc));
+  return SKX_OK;
+}
+
+int skx_create(const skx_desc* desc, skx_handle* out) {
+  if (!out) return fail(SKX_EINVAL, "null output handle");
+  return create_engine(desc, nullptr, nullptr, nullptr, 0, out);
 }
 
 int skx_destroy(skx_handle h) {
@@ -732,6 +762,333 @@ int skx_trace_reset(skx_handle h) {
 int skx_sync(skx_handle h) {
   CHECK_HANDLE(h);
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return SKX_OK;
+}
+
+}  // extern "C"
+
+// ====================================================================== batches
+// A batch runs many lattices of one theory / precision / tile shape with one
+// launch per RK4 stage.  Every lattice (slot) is a full handle (its own
+// buffers, partials and status) that shares the batch's stream; its status
+// lives in one contiguous device array so a poll is a single copy.  A block
+// table maps each block of the fixed-size grid to a slot, so lattices join
+// and leave between chunks without re-capturing the step graph: only the
+// tables are re-uploaded.  Chunks have an even number of steps and every
+// lattice joins at a chunk boundary, so all lattices share the ping-pong
+// parity of the graph's launches.
+
+struct skx_batch {
+  int theory = 0, prec = 0, device = 0;
+  uint32_t flags = 0;
+  LaunchShape tile{};
+  int max_slots = 0, max_blocks = 0;
+  hipStream_t stream = nullptr;
+  std::vector<skx_engine*> slot;
+  std::vector<int> first_block, nblocks;
+  std::vector<int> h_blk;
+  void* d_slots = nullptr;  // BatchSlot<T>[max_slots]
+  void* h_slots = nullptr;  // pinned mirror
+  int* d_blk = nullptr;
+  DevStatus* d_status = nullptr;
+  DevStatus* h_status = nullptr;
+  bool dirty = true;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+};
+
+namespace {
+
+size_t slot_bytes(int prec) { return prec == SKX_FP64 ? sizeof(BatchSlot<double>) : sizeof(BatchSlot<float>); }
+
+template <typename T>
+void fill_slot(skx_batch* b, int i) {
+  BatchSlot<T>* tab = static_cast<BatchSlot<T>*>(b->h_slots);
+  BatchSlot<T>& sl = tab[i];
+  std::memset(&sl, 0, sizeof(sl));
+  skx_engine* e = b->slot[i];
+  if (!e) return;
+  const int fl[4] = {FUSE_ENERGY | DECIDE_ARREST | CHECK_DONE, CHECK_DONE, CHECK_DONE, RUN_MODE | CHECK_DONE};
+  for (int stage = 1; stage <= 4; ++stage)
+    for (int par = 0; par < 2; ++par) {
+      StageArgs<T> a = make_args<T>(e, stage, fl[stage - 1], par);
+      a.nbz = (e->L.Z + b->tile.tz - 1) / b->tile.tz;
+      a.nby = (e->L.Y + b->tile.ty - 1) / b->tile.ty;
+      a.nbx = (e->L.X + a.xchunk - 1) / a.xchunk;
+      sl.args[stage - 1][par] = a;
+    }
+  sl.first_block = b->first_block[i];
+  sl.nblocks = b->nblocks[i];
+  sl.active = 1;
+}
+
+int upload_tables(skx_batch* b) {
+  if (!b->dirty) return SKX_OK;
+  for (int i = 0; i < b->max_slots; ++i) {
+    if (b->prec == SKX_FP64) fill_slot<double>(b, i);
+    else fill_slot<float>(b, i);
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_slots, b->h_slots, slot_bytes(b->prec) * b->max_slots, hipMemcpyHostToDevice,
+                         b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_blk, b->h_blk.data(), sizeof(int) * b->max_blocks, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->dirty = false;
+  return SKX_OK;
+}
+
+hipError_t enqueue_batch_step(skx_batch* b, int parity) {
+  for (int stage = 1; stage <= 4; ++stage) {
+    hipError_t r = b->prec == SKX_FP64
+                       ? launch_stage_batch_f64(b->theory, stage, b->tile,
+                                                static_cast<const BatchSlot<double>*>(b->d_slots), b->d_blk, parity,
+                                                b->max_blocks, b->stream)
+                       : launch_stage_batch_f32(b->theory, stage, b->tile,
+                                                static_cast<const BatchSlot<float>*>(b->d_slots), b->d_blk, parity,
+                                                b->max_blocks, b->stream);
+    if (r != hipSuccess) return r;
+  }
+  return hipSuccess;
+}
+
+int batch_graph(skx_batch* b, int steps) {
+  if (b->graph && b->graph_steps == steps) return SKX_OK;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t r = hipSuccess;
+  for (int s = 0; s < steps && r == hipSuccess; ++s) r = enqueue_batch_step(b, s & 1);
+  hipError_t r2 = hipStreamEndCapture(b->stream, &g);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("batched launch during capture: ") + hipGetErrorString(r));
+  if (r2 != hipSuccess) return fail(SKX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(r2));
+  hipError_t r3 = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (r3 != hipSuccess) {
+    b->graph = nullptr;
+    return fail(SKX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r3));
+  }
+  b->graph_steps = steps;
+  return SKX_OK;
+}
+
+void release_batch(skx_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (skx_engine* e : b->slot)
+    if (e) release(e);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->d_slots) (void)hipFree(b->d_slots);
+  if (b->h_slots) (void)hipHostFree(b->h_slots);
+  if (b->d_blk) (void)hipFree(b->d_blk);
+  if (b->d_status) (void)hipFree(b->d_status);
+  if (b->h_status) (void)hipHostFree(b->h_status);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+void slot_status(skx_batch* b, int i, skx_slot_status* o) {
+  const DevStatus& s = b->h_status[i];
+  o->slot = i;
+  o->active = b->slot[i] ? 1 : 0;
+  o->steps = s.steps;
+  o->done = s.done;
+  o->converged = (s.steps > 0 && s.err_abs < s.tol) ? 1 : 0;
+  o->nonfinite = s.nonfinite;
+  o->energy = s.energy;
+  o->err_signed = s.err_signed;
+  o->err_abs = s.err_abs;
+  o->baryon = s.baryon;
+}
+
+#define CHECK_BATCH(b)                                                         \
+  do {                                                                         \
+    if (!(b)) return fail(SKX_EINVAL, "null batch");                           \
+    if (hipSetDevice((b)->device) != hipSuccess) return fail(SKX_EHIP, "hipSetDevice failed"); \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
+  if (!d || !out) return fail(SKX_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->theory != SKX_THEORY_PION && d->theory != SKX_THEORY_OMEGA && d->theory != SKX_THEORY_COULOMB)
+    return fail(SKX_ENOTSUP, "batches run the pion, omega and Coulomb theories");
+  if (d->precision != SKX_FP64 && d->precision != SKX_FP32) return fail(SKX_EINVAL, "unknown precision");
+  if (d->max_slots < 1 || d->max_blocks < 1) return fail(SKX_EINVAL, "max_slots and max_blocks must be positive");
+  skx_batch* b = new skx_batch();
+  b->theory = d->theory;
+  b->prec = d->precision;
+  b->device = d->device;
+  b->flags = d->flags;
+  b->tile = d->tile_z == 64 ? LaunchShape{64, 4, 1} : (d->tile_z == 16 ? LaunchShape{16, 16, 1} : LaunchShape{32, 8, 1});
+  b->max_slots = d->max_slots;
+  b->max_blocks = d->max_blocks;
+  b->slot.assign(b->max_slots, nullptr);
+  b->first_block.assign(b->max_slots, 0);
+  b->nblocks.assign(b->max_slots, 0);
+  b->h_blk.assign(b->max_blocks, -1);
+  hipError_t r;
+  if ((r = hipSetDevice(b->device)) != hipSuccess) {
+    delete b;
+    return fail(SKX_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(r));
+  }
+  if ((r = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) {
+    b->stream = nullptr;
+    release_batch(b);
+    return fail(SKX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(r));
+  }
+  const size_t sb = slot_bytes(b->prec) * b->max_slots;
+  if (hipMalloc(&b->d_slots, sb) != hipSuccess || hipHostMalloc(&b->h_slots, sb, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&b->d_blk, sizeof(int) * b->max_blocks) != hipSuccess ||
+      hipMalloc(&b->d_status, sizeof(DevStatus) * b->max_slots) != hipSuccess ||
+      hipHostMalloc(&b->h_status, sizeof(DevStatus) * b->max_slots, hipHostMallocDefault) != hipSuccess) {
+    release_batch(b);
+    return fail(SKX_ENOMEM, "batch table allocation failed");
+  }
+  std::memset(b->h_status, 0, sizeof(DevStatus) * b->max_slots);
+  (void)launch_fill(b->d_status, sizeof(DevStatus) * b->max_slots, b->stream);
+  *out = b;
+  return SKX_OK;
+}
+
+int skx_batch_destroy(skx_batch_handle b) {
+  if (!b) return SKX_OK;
+  release_batch(b);
+  return SKX_OK;
+}
+
+int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0, const double* V0, int32_t max_steps,
+                  double tol, int32_t* slot_out) {
+  CHECK_BATCH(b);
+  if (!lattice || !F0 || !slot_out) return fail(SKX_EINVAL, "null argument");
+  if (lattice->theory != b->theory || lattice->precision != b->prec)
+    return fail(SKX_EINVAL, "lattice theory / precision differ from the batch's");
+  if (max_steps < 0) return fail(SKX_EINVAL, "max_steps must be >= 0");
+  int i = 0;
+  while (i < b->max_slots && b->slot[i]) ++i;
+  if (i == b->max_slots) return fail(SKX_EBUSY, "no free slot in the batch");
+  Lattice L;
+  if (int rc = parse_desc(lattice, L)) return rc;
+  // one x-chunk per lattice: the batch's other lattices supply the parallelism
+  const int nblk = (int)(((L.Z + b->tile.tz - 1) / b->tile.tz) * (long long)((L.Y + b->tile.ty - 1) / b->tile.ty));
+  int first = -1;
+  for (int s = 0, run = 0; s < b->max_blocks; ++s) {  // first fit in the block table
+    run = (b->h_blk[s] < 0) ? run + 1 : 0;
+    if (run == nblk) {
+      first = s - nblk + 1;
+      break;
+    }
+  }
+  if (first < 0) return fail(SKX_EBUSY, "not enough free blocks in the batch grid");
+  skx_desc d = *lattice;
+  d.tile_z = b->tile.tz;
+  skx_engine* e = nullptr;
+  if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, L.X, &e)) return rc;
+  const size_t n = (size_t)L.N * L.nf;
+  if (int rc = copy_in(e, e->F[0], F0, n)) {
+    release(e);
+    return rc;
+  }
+  if (V0) {
+    if (int rc = copy_in(e, e->V[0], V0, n)) {
+      release(e);
+      return rc;
+    }
+  }
+  DevStatus* h = b->h_status + i;
+  std::memset(h, 0, sizeof(DevStatus));
+  h->keep = 1.0;
+  h->tol = tol;
+  h->max_steps = max_steps;
+  h->killkinen = L.killkinen;
+  if (int rc = push_status(e)) {
+    release(e);
+    return rc;
+  }
+  b->slot[i] = e;
+  b->first_block[i] = first;
+  b->nblocks[i] = nblk;
+  for (int s = first; s < first + nblk; ++s) b->h_blk[s] = i;
+  b->dirty = true;
+  *slot_out = i;
+  return SKX_OK;
+}
+
+int skx_batch_launch(skx_batch_handle b, int32_t steps) {
+  CHECK_BATCH(b);
+  if (steps < 0) return fail(SKX_EINVAL, "steps must be >= 0");
+  if (int rc = upload_tables(b)) return rc;
+  const int chunk = 2 * std::max(1, (int)((steps + 1) / 2));
+  if (!(b->flags & SKX_FLAG_NO_GRAPH)) {
+    if (int rc = batch_graph(b, chunk)) return rc;
+    HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+  } else {
+    for (int s = 0; s < chunk; ++s) {
+      hipError_t r = enqueue_batch_step(b, s & 1);
+      if (r != hipSuccess) return fail(SKX_EHIP, std::string("batched launch: ") + hipGetErrorString(r));
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(b->h_status, b->d_status, sizeof(DevStatus) * b->max_slots, hipMemcpyDeviceToHost,
+                         b->stream));
+  return SKX_OK;
+}
+
+int skx_batch_wait(skx_batch_handle b) {
+  CHECK_BATCH(b);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return SKX_OK;
+}
+
+int skx_batch_run(skx_batch_handle b, int32_t steps) {
+  if (int rc = skx_batch_launch(b, steps)) return rc;
+  return skx_batch_wait(b);
+}
+
+int skx_batch_poll(skx_batch_handle b, skx_slot_status* out, int32_t n, int32_t* n_out) {
+  CHECK_BATCH(b);
+  if (!out || !n_out) return fail(SKX_EINVAL, "null argument");
+  int k = 0;
+  for (int i = 0; i < b->max_slots && k < n; ++i)
+    if (b->slot[i]) slot_status(b, i, &out[k++]);
+  *n_out = k;
+  return SKX_OK;
+}
+
+int skx_batch_remove(skx_batch_handle b, int32_t slot, double* F_out, double* V_out, skx_slot_status* final_status) {
+  CHECK_BATCH(b);
+  if (slot < 0 || slot >= b->max_slots || !b->slot[slot]) return fail(SKX_EINVAL, "no lattice in that slot");
+  skx_engine* e = b->slot[slot];
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (int rc = pull_status(e)) return rc;
+  // the lattice joined at parity 0 and has taken `steps` steps
+  e->cur = e->st_host->steps & 1;
+  if (e->st_host->steps > 0) {
+    hipError_t r = launch(e, 0, FUSE_ENERGY | FUSE_BARYON | DECIDE_ARREST, e->cur);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("energy pass: ") + hipGetErrorString(r));
+    const long long n = e->L.N * e->L.nf;
+    r = e->L.prec == SKX_FP64 ? launch_apply_arrest_f64(static_cast<double*>(e->V[e->cur]), n, e->st, e->stream)
+                              : launch_apply_arrest_f32(static_cast<float*>(e->V[e->cur]), n, e->st, e->stream);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("arrest: ") + hipGetErrorString(r));
+    if (int rc = pull_status(e)) return rc;
+  }
+  if (final_status) slot_status(b, slot, final_status);
+  const size_t n = (size_t)e->L.N * e->L.nf;
+  if (F_out) {
+    if (int rc = copy_out(e, F_out, e->F[e->cur], n)) return rc;
+  }
+  if (V_out) {
+    if (int rc = copy_out(e, V_out, e->V[e->cur], n)) return rc;
+  }
+  for (int s = b->first_block[slot]; s < b->first_block[slot] + b->nblocks[slot]; ++s) b->h_blk[s] = -1;
+  b->slot[slot] = nullptr;
+  release(e);
+  std::memset(b->h_status + slot, 0, sizeof(DevStatus));
+  b->dirty = true;
   return SKX_OK;
 }
 

@@ -1,0 +1,579 @@
+// Stage kernel templates of the arrested-Newton-flow step for gfx950
+// (pion, omega and Coulomb theories).
+//
+// One kernel template covers the energy pass (stage 0) and the four RK4
+// gradient stages of one step.  Each stage is one pass over the lattice:
+//
+//   stage field   Temp_s = F + f_s * dt * (V + f'_s * l_{s-1})   (formed on load)
+//   stage 1       l1 = -dt g(Temp_1)            [+ energy / baryon of F fused]
+//   stage 2       l2 = -dt g(Temp_2)
+//   stage 3       l3 = -dt g(Temp_3)
+//   stage 4       l4 = -dt g(Temp_4); F', V' = RK4 combine; S^3 normalise and
+//                 tangent projection; signed-max and max|g| reductions fused
+//
+// which is the data flow of core/integrator.py:382-409 (k_s and Temp are
+// never stored; they are recomputed from F, V and the l_s on the fly).
+//
+// Tiling: lanes run along the unit-stride axis (reference z), a block covers
+// a TZ x TY tile of the (y, z) plane and marches along the slowest axis
+// (reference x) over a chunk of planes.  Each incoming plane of the stage
+// field (with a 2-site y/z halo) is staged in a 4-slot LDS ring, so the
+// centre plane (two behind the lead) is still resident for its y/z second
+// differences; one barrier per plane.  Per-site values along x (the field
+// and its raw y / z first differences) live in 5-deep register queues, and
+// raw z-differences of every tile row go to a 3-slot LDS ring for the yz
+// mixed derivative.  Mixed derivatives are first differences of raw first
+// differences, which is the reference's 25-tap tensor stencil
+// (core/derivatives.py:121-173) regrouped.
+#pragma once
+
+#include "skx_device.h"
+#include "skx_rho_curl.h"
+
+namespace skx {
+
+template <int TH>
+struct Traits;
+// NF components in all; the first NST take the stencil (the others are read
+// at the site only); MIXED marks the stencil components needing mixed second
+// derivatives; NAUX per-site values come from the rho curl pass.
+template <>
+struct Traits<PION> {
+  static constexpr int NF = 4, NST = 4, NSK = 4, NAUX = 0;
+  static constexpr unsigned MIXED = 0xFu;
+};
+template <>
+struct Traits<OMEGA> {
+  static constexpr int NF = 5, NST = 5, NSK = 4, NAUX = 0;
+  static constexpr unsigned MIXED = 0x0u;
+};
+template <>
+struct Traits<COULOMB> {
+  static constexpr int NF = 5, NST = 5, NSK = 4, NAUX = 0;
+  static constexpr unsigned MIXED = 0xFu;
+};
+// rho: the 4 pion components take the stencil; the 9 rho components enter the
+// gradient only through D^a_j and the energy only through R^a_ij, both
+// formed by the curl pass (k_rho_curl) and read here per site.
+template <>
+struct Traits<RHO> {
+  static constexpr int NF = 13, NST = 4, NSK = 4, NAUX = 9;
+  static constexpr unsigned MIXED = 0xFu;
+};
+
+template <int TH, typename T, int LD>
+__device__ __forceinline__ void point_gradient(const T* phi, const T (*F)[LD], const T (*S)[LD], const T (*aux)[3],
+                                               const PhysParams& P, T* g) {
+  if constexpr (TH == PION) gradient_pion<T>(phi, F, S, P, g);
+  if constexpr (TH == OMEGA) gradient_omega<T>(phi, F, S, P, g);
+  if constexpr (TH == COULOMB) gradient_coulomb<T>(phi, F, S, P, g);
+  if constexpr (TH == RHO) gradient_rho_split<T>(phi, F, S, aux, P, g);
+}
+
+template <int TH, typename T, int LD>
+__device__ __forceinline__ double point_energy(const T* phi, const T (*F)[LD], const T (*aux)[3],
+                                               const PhysParams& P) {
+  if constexpr (TH == PION) return energy_pion<T, LD>(phi, F, P);
+  if constexpr (TH == OMEGA) return energy_omega<T, LD>(phi, F, P);
+  if constexpr (TH == COULOMB) return energy_coulomb<T, LD>(phi, F, P);
+  if constexpr (TH == RHO) return energy_rho_split<T>(phi, F, aux, P);
+  return 0.0;
+}
+
+// One stage over the block `blk` of `nblk` blocks that cover one lattice.
+template <typename T, int TH, int STAGE, int TZ, int TY>
+__device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int nblk) {
+  using Tr = Traits<TH>;
+  constexpr int NF = Tr::NF;
+  constexpr int NST = Tr::NST;
+  constexpr int NSK = Tr::NSK;
+  constexpr int NAUX = Tr::NAUX;
+  constexpr int NT = TZ * TY;
+  constexpr int TW = TZ + 4;  // tile row length (z with halo)
+  constexpr int TR = TY + 4;  // tile rows (y with halo)
+  constexpr int TE = TW * TR;
+  constexpr int LPT = (TE + NT - 1) / NT;
+  constexpr int NA = (STAGE <= 1) ? 1 : (STAGE == 2 ? 2 : 3);
+  constexpr bool D2 = STAGE >= 1;
+  constexpr bool ANY_MIXED = D2 && (Tr::MIXED != 0u);
+  constexpr int FSLOTS = 4;  // stage-field planes: lead p, p-1, centre p-2, and the slot being refilled
+  constexpr int ZSLOTS = 3;  // raw z-differences for the yz mixed derivative: planes p, p-1, p-2
+  static_assert(NT % 64 == 0, "block must be whole wavefronts");
+  static_assert(TY >= 4, "halo rows of the z-difference ring are filled by the first four rows of threads");
+
+  __shared__ T fring[FSLOTS][NST][TE];
+  __shared__ T zring[ANY_MIXED ? ZSLOTS : 1][ANY_MIXED ? NST : 1][ANY_MIXED ? TR * TZ : 1];
+  __shared__ double red[2 * (NT / 64)];
+
+  if ((A.flags & CHECK_DONE) && A.st->done) return;  // uniform per launch
+
+  const int tid = threadIdx.x;
+  const int tz = tid % TZ, ty = tid / TZ;
+  // XCD-aware order: consecutive tiles (which share y halos) on the same L2
+  int lin = blk;
+  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
+  const int bz = lin % A.nbz;
+  lin /= A.nbz;
+  const int by = lin % A.nby;
+  const int bx = lin / A.nby;
+  const int z0 = bz * TZ, y0 = by * TY;
+  const int xs = bx * A.xchunk;
+  const int xe = min(xs + A.xchunk, A.X);
+  const int y = y0 + ty, z = z0 + tz;
+  const int Y = A.Y, Z = A.Z, X = A.X, halo = A.halo;
+  const bool own = (y < Y) && (z < Z);
+  const size_t N = (size_t)A.N;
+  const size_t YZ = (size_t)Y * Z;
+
+  const bool gy1 = (y >= halo) && (y < Y - halo), gz1 = (z >= halo) && (z < Z - halo);
+  const bool gy2 = (y > halo) && (y < Y - halo), gz2 = (z > halo) && (z < Z - halo);
+
+  const T dt = T(A.P.dt);
+  T keep = T(1);
+  if constexpr (STAGE >= 2) keep = T(A.st->keep);
+  AxisCoef<T> cx, cy, cz;
+  cx.set(A.ilsx);
+  cy.set(A.ilsy);
+  cz.set(A.ilsz);
+  // per-thread guarded coefficients (the guards of core/derivatives.py:54,106,121 along y and z)
+  const T y2a = gy1 ? cy.d2a : T(0), y2b = gy1 ? cy.d2b : T(0), y2c = gy1 ? cy.d2c : T(0);
+  const T z2a = gz1 ? cz.d2a : T(0), z2b = gz1 ? cz.d2b : T(0), z2c = gz1 ? cz.d2c : T(0);
+  const T mxy_a = gy2 ? cx.d1a : T(0), mxy_b = gy2 ? cx.d1b : T(0);  // x-part of d_x d_y, y guard folded
+  const T mxz_a = gz2 ? cx.d1a : T(0), mxz_b = gz2 ? cx.d1b : T(0);
+  const T myz_a = (gy2 && gz2) ? cy.d1a : T(0), myz_b = (gy2 && gz2) ? cy.d1b : T(0);
+
+  // tile elements this thread stages: byte offsets within a plane, kOOB for
+  // cells outside the lattice (they load as 0)
+  uint32_t eoff[LPT];
+#pragma unroll
+  for (int j = 0; j < LPT; ++j) {
+    const int e = tid + j * NT;
+    const int r = e / TW, cc = e - r * TW;
+    const int gy = y0 - 2 + r, gz = z0 - 2 + cc;
+    const bool valid = (e < TE) && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
+    eoff[j] = valid ? (uint32_t)(gy * Z + gz) * (uint32_t)sizeof(T) : kOOB;
+  }
+  const uint32_t own_off = own ? (uint32_t)(y * Z + z) : 0u;
+  const uint32_t own_b = own ? own_off * (uint32_t)sizeof(T) : kOOB;
+  const uint32_t plane_bytes = (uint32_t)(YZ * sizeof(T));
+  // halo row of the z-difference ring this thread fills (rows 0,1 and TY+2,TY+3)
+  const int hrow = (ty < 2) ? ty : (TY + ty);
+
+  const T* src[3] = {A.F, A.V, A.la};
+  T pre[NA][NST][LPT];
+  auto issue = [&](int p) {
+    if (p < 0 || p >= X) {  // uniform: planes outside the lattice are zero
+#pragma unroll
+      for (int k = 0; k < NA; ++k)
+#pragma unroll
+        for (int a = 0; a < NST; ++a)
+#pragma unroll
+          for (int j = 0; j < LPT; ++j) pre[k][a][j] = T(0);
+      return;
+    }
+#pragma unroll
+    for (int k = 0; k < NA; ++k)
+#pragma unroll
+      for (int a = 0; a < NST; ++a) {
+        const auto r = plane_rsrc(src[k] + a * N + (size_t)p * YZ, plane_bytes);
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) pre[k][a][j] = bload<T>(r, eoff[j]);
+      }
+  };
+
+  // own-column x-queues, 5 slots rotated by the unrolled plane loop
+  T qf[NST][5], qy[NST][5], qz[NST][5];
+#pragma unroll
+  for (int a = 0; a < NST; ++a)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) qf[a][k] = qy[a][k] = qz[a][k] = T(0);
+
+  double acc0 = (STAGE == 4) ? -1.0e300 : 0.0;
+  double acc1 = 0.0;
+  bool bad = false;
+  const int C = (ty + 2) * TW + (tz + 2);
+
+  // One plane of the march.  K = phase mod 5: the lead plane p goes to queue
+  // slot K; the centre c = p - 2 and its x-neighbours c-2..c+2 sit in slots
+  // (K+1)..(K+5) mod 5.
+  auto body = [&](auto Kc, int p) {
+    constexpr int K = decltype(Kc)::value;
+    constexpr int sm2 = (K + 1) % 5, sm1 = (K + 2) % 5, s0 = (K + 3) % 5, sp1 = (K + 4) % 5, sp2 = K;
+    const int fs = (p + 8) & (FSLOTS - 1);  // p >= -2
+#pragma unroll
+    for (int a = 0; a < NST; ++a)
+#pragma unroll
+      for (int j = 0; j < LPT; ++j) {
+        const int e = tid + j * NT;
+        if (j < LPT - 1 || e < TE)
+          fring[fs][a][e] = stage_value<STAGE>(pre[0][a][j], NA > 1 ? pre[NA > 1 ? 1 : 0][a][j] : T(0),
+                                               NA > 2 ? pre[NA > 2 ? 2 : 0][a][j] : T(0), dt, keep);
+      }
+    __syncthreads();
+    if (p + 1 <= xe + 1) issue(p + 1);
+
+    const int c = p - 2;
+    const bool center = (c >= xs) && own;
+    // stage 4 reads its own site's F, V, l1, l2, l3 for the RK4 combine: issue early
+    T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
+    if constexpr (STAGE == 4) {
+      if (c >= xs) {
+        const size_t pb = (size_t)c * YZ;
+#pragma unroll
+        for (int a = 0; a < NF; ++a) {
+          const size_t o = a * N + pb;
+          cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), own_b);
+          cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), own_b);
+          cl1[a] = bload<T>(plane_rsrc(A.lb + o, plane_bytes), own_b);
+          cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), own_b);
+          cl3[a] = bload<T>(plane_rsrc(A.lc + o, plane_bytes), own_b);
+        }
+      }
+    }
+    // site-only components (stages 0-3; stage 4 has them in cF / cV / cl2) and the curl-pass values
+    T cP[NA][NF > NST ? NF - NST : 1];
+    T cx_aux[NAUX > 0 ? NAUX : 1];
+    if constexpr (NF > NST || NAUX > 0) {
+      if (c >= xs) {
+        const size_t pb = (size_t)c * YZ;
+        if constexpr (NF > NST && STAGE <= 3) {
+#pragma unroll
+          for (int k = 0; k < NA; ++k)
+#pragma unroll
+            for (int a = NST; a < NF; ++a)
+              cP[k][a - NST] = bload<T>(plane_rsrc(src[k] + a * N + pb, plane_bytes), own_b);
+        }
+        if constexpr (NAUX > 0) {
+          // stages >= 1: D^a_j at aux[3a + j]; stages 0 / 1 with the energy fused: R^a_ij at aux[9 + 3a + p]
+          if constexpr (STAGE >= 1) {
+#pragma unroll
+            for (int q = 0; q < NAUX; ++q) cx_aux[q] = bload<T>(plane_rsrc(A.aux + q * N + pb, plane_bytes), own_b);
+          }
+        }
+      }
+    }
+
+    // lead plane p: own-column value and raw first differences along y and z
+    const int zs = (p + 9) % ZSLOTS;
+#pragma unroll
+    for (int a = 0; a < NST; ++a) {
+      const T* t = fring[fs][a];
+      qf[a][sp2] = t[C];
+      qy[a][sp2] = cd1(t[C - 2 * TW], t[C - TW], t[C + TW], t[C + 2 * TW], cy.d1a, cy.d1b);
+      const T dz = cd1(t[C - 2], t[C - 1], t[C + 1], t[C + 2], cz.d1a, cz.d1b);
+      qz[a][sp2] = dz;
+      if constexpr (ANY_MIXED) {
+        if ((Tr::MIXED >> a) & 1u) {
+          zring[zs][a][(ty + 2) * TZ + tz] = dz;
+          if (ty < 4) {
+            const int h = hrow * TW + tz + 2;
+            zring[zs][a][hrow * TZ + tz] = cd1(t[h - 2], t[h - 1], t[h + 1], t[h + 2], cz.d1a, cz.d1b);
+          }
+        }
+      }
+    }
+
+    if (center) {
+      // x guards are uniform over the block for this plane
+      const bool gx1 = (c >= halo) && (c < X - halo), gx2 = (c > halo) && (c < X - halo);
+      const T x1a = gx1 ? cx.d1a : T(0), x1b = gx1 ? cx.d1b : T(0);
+      const T x2a = gx1 ? cx.d2a : T(0), x2b = gx1 ? cx.d2b : T(0), x2c = gx1 ? cx.d2c : T(0);
+      const T xya = gx2 ? mxy_a : T(0), xyb = gx2 ? mxy_b : T(0);
+      const T xza = gx2 ? mxz_a : T(0), xzb = gx2 ? mxz_b : T(0);
+      const int cfs = (c + 8) & (FSLOTS - 1);
+      const int czs = (c + 9) % ZSLOTS;
+      T phi[NF], F[3][NST], S[6][NST];
+#pragma unroll
+      for (int a = 0; a < NST; ++a) {
+        const T f0 = qf[a][s0];
+        phi[a] = f0;
+        F[0][a] = cd1(qf[a][sm2], qf[a][sm1], qf[a][sp1], qf[a][sp2], x1a, x1b);
+        F[1][a] = gy1 ? qy[a][s0] : T(0);
+        F[2][a] = gz1 ? qz[a][s0] : T(0);
+        if constexpr (D2) {
+          const T* t = fring[cfs][a];
+          S[0][a] = cd2(qf[a][sm2], qf[a][sm1], f0, qf[a][sp1], qf[a][sp2], x2a, x2b, x2c);
+          S[1][a] = cd2(t[C - 2 * TW], t[C - TW], f0, t[C + TW], t[C + 2 * TW], y2a, y2b, y2c);
+          S[2][a] = cd2(t[C - 2], t[C - 1], f0, t[C + 1], t[C + 2], z2a, z2b, z2c);
+          if ((Tr::MIXED >> a) & 1u) {
+            S[3][a] = cd1(qy[a][sm2], qy[a][sm1], qy[a][sp1], qy[a][sp2], xya, xyb);
+            S[4][a] = cd1(qz[a][sm2], qz[a][sm1], qz[a][sp1], qz[a][sp2], xza, xzb);
+            if constexpr (ANY_MIXED) {
+              const T* zr = zring[czs][a];
+              S[5][a] = cd1(zr[ty * TZ + tz], zr[(ty + 1) * TZ + tz], zr[(ty + 3) * TZ + tz], zr[(ty + 4) * TZ + tz],
+                            myz_a, myz_b);
+            }
+          } else {
+            S[3][a] = S[4][a] = S[5][a] = T(0);
+          }
+        }
+      }
+      if constexpr (NF > NST) {
+#pragma unroll
+        for (int a = NST; a < NF; ++a) {
+          if constexpr (STAGE == 4) phi[a] = stage_value<4>(cF[a], cV[a], cl2[a], dt, keep);
+          else
+            phi[a] = stage_value<STAGE>(cP[0][a - NST], NA > 1 ? cP[NA > 1 ? 1 : 0][a - NST] : T(0),
+                                        NA > 2 ? cP[NA > 2 ? 2 : 0][a - NST] : T(0), dt, keep);
+        }
+      }
+      const size_t so = (size_t)c * YZ + own_off;
+
+      if constexpr (STAGE <= 1) {
+        if (STAGE == 0 || (A.flags & FUSE_ENERGY)) {
+          T Rv[3][3];
+          if constexpr (NAUX > 0) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+              Rv[q / 3][q % 3] = bload<T>(plane_rsrc(A.aux + (9 + q) * N + (size_t)c * YZ, plane_bytes), own_b);
+          }
+          const double e = point_energy<TH, T, NST>(phi, F, Rv, A.P);
+          acc0 += e;
+          if (STAGE == 0 && A.dens_e) A.dens_e[so] = e;
+        }
+        if (A.flags & FUSE_BARYON) {
+          const double b = baryon_density<T, NST>(phi, F, A.P);
+          acc1 += b;
+          if (STAGE == 0 && A.dens_b) A.dens_b[so] = b;
+        }
+      }
+      if constexpr (D2) {
+        T g[NF];
+        T Dv[3][3];
+        if constexpr (NAUX > 0) {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) Dv[q / 3][q % 3] = cx_aux[q];
+        }
+        point_gradient<TH, T, NST>(phi, F, S, Dv, A.P, g);
+        if (A.eg_out) {
+#pragma unroll
+          for (int a = 0; a < NF; ++a) A.eg_out[a * N + so] = double(g[a]);
+        }
+        if constexpr (STAGE <= 3) {
+#pragma unroll
+          for (int a = 0; a < NF; ++a) bstore<T>(-dt * g[a], plane_rsrc(A.lout + a * N + (size_t)c * YZ, plane_bytes), own_b);
+        } else {
+          // RK4 combine (core/integrator.py:170-175), recomputing k1..k4 from V and l1..l3
+          using R = double;
+          R Fn[NF], Vn[NF];
+          const R rdt = R(A.P.dt);
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            const R Fv = R(cF[a]);
+            const R Vk = (keep != T(0)) ? R(cV[a]) : R(0);
+            const R l1 = R(cl1[a]), l2 = R(cl2[a]), l3 = R(cl3[a]);
+            const R l4 = R(-dt * g[a]);
+            const R k1 = rdt * Vk;
+            const R k2 = rdt * (Vk + 0.5 * l1);
+            const R k3 = rdt * (Vk + 0.5 * l2);
+            const R k4 = rdt * (Vk + l3);
+            Vn[a] = Vk + (1.0 / 6.0) * (l1 + 2.0 * l2 + 2.0 * l3 + l4);
+            Fn[a] = Fv + (1.0 / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+            const double gv = double(g[a]);
+            acc0 = fmax(acc0, gv);
+            acc1 = fmax(acc1, fabs(gv));
+            bad |= !isfinite(gv);
+          }
+          if (A.unit_length) {
+            R s = 0.0;
+#pragma unroll
+            for (int a = 0; a < NSK; ++a) s += Fn[a] * Fn[a];
+            s = sqrt(s);
+            if (s != 0.0) {
+#pragma unroll
+              for (int a = 0; a < NSK; ++a) Fn[a] /= s;
+            }
+            R lm = 0.0;
+#pragma unroll
+            for (int a = 0; a < NSK; ++a) lm += Vn[a] * Fn[a];
+#pragma unroll
+            for (int a = 0; a < NSK; ++a) Vn[a] -= lm * Fn[a];
+          }
+#pragma unroll
+          for (int a = 0; a < NF; ++a) {
+            bstore<T>(T(Fn[a]), plane_rsrc(A.Fout + a * N + (size_t)c * YZ, plane_bytes), own_b);
+            bstore<T>(T(Vn[a]), plane_rsrc(A.Vout + a * N + (size_t)c * YZ, plane_bytes), own_b);
+          }
+        }
+      }
+    }
+  };
+
+  issue(xs - 2);
+  const int pend = xe + 1;
+  for (int p = xs - 2;;) {
+    body(IC<0>{}, p);
+    if (++p > pend) break;
+    body(IC<1>{}, p);
+    if (++p > pend) break;
+    body(IC<2>{}, p);
+    if (++p > pend) break;
+    body(IC<3>{}, p);
+    if (++p > pend) break;
+    body(IC<4>{}, p);
+    if (++p > pend) break;
+  }
+
+  // ---- fused reductions
+  if constexpr (STAGE <= 1) {
+    if (STAGE == 1 && !(A.flags & (FUSE_ENERGY | FUSE_BARYON))) return;
+    double r0 = acc0, r1 = acc1;
+    if (grid_reduce2<NT, false>(r0, r1, A.partial, &A.st->cnt_sum, red, blk, nblk)) {
+      DevStatus* st = A.st;
+      if (A.flags & FUSE_ENERGY) st->energy = r0;
+      if (A.flags & FUSE_BARYON) st->baryon = r1;
+      if (!isfinite(r0) || !isfinite(r1)) {
+        st->nonfinite = 1;
+        st->done = 1;
+      }
+      if (A.flags & DECIDE_ARREST) {
+        double kp = 1.0;
+        if (st->arrest_pending && st->killkinen && (r0 > st->e_prev)) kp = 0.0;
+        st->keep = kp;
+        st->arrest_pending = 0;
+        st->e_prev = r0;
+      }
+      if (A.hist && (A.flags & DECIDE_ARREST) && st->steps < st->hist_cap) {
+        A.hist[st->steps].energy = r0;
+        A.hist[st->steps].baryon = r1;
+      }
+    }
+  } else if constexpr (STAGE == 4) {
+    // +inf in the max|g| slot marks a non-finite gradient
+    double r0 = acc0, r1 = bad ? __longlong_as_double(0x7ff0000000000000LL) : acc1;
+    if (grid_reduce2<NT, true>(r0, r1, A.partial, &A.st->cnt_max, red, blk, nblk)) {
+      DevStatus* st = A.st;
+      st->err_signed = r0;
+      st->err_abs = r1;
+      if (!isfinite(r1)) {
+        st->nonfinite = 1;
+        st->done = 1;
+      }
+      if (A.flags & RUN_MODE) {
+        if (A.hist && st->steps < st->hist_cap) {
+          A.hist[st->steps].err_signed = r0;
+          A.hist[st->steps].err_abs = r1;
+        }
+        st->steps += 1;
+        st->arrest_pending = 1;
+        if (r1 < st->tol || st->steps >= st->max_steps) st->done = 1;
+      }
+    }
+  }
+}
+
+template <typename T, int TH, int STAGE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ* TY) k_stage(const StageArgs<T> A) {
+  stage_body<T, TH, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
+}
+
+// Batched launch: many lattices of one theory / precision / tile shape in one
+// grid.  blk_inst maps every block of the grid to a slot (-1 = idle block);
+// the slot carries the lattice's arguments for both ping-pong parities.
+template <typename T, int TH, int STAGE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ* TY) k_stage_batch(const BatchSlot<T>* __restrict__ slots,
+                                                       const int* __restrict__ blk_inst, int parity) {
+  const int inst = blk_inst[blockIdx.x];
+  if (inst < 0) return;
+  const BatchSlot<T>& sl = slots[inst];
+  if (!sl.active) return;
+  if constexpr (STAGE >= 1)
+    stage_body<T, TH, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], (int)blockIdx.x - sl.first_block, sl.nblocks);
+}
+
+// ------------------------------------------------------------------ per-theory launch
+// Compile-time visit of (stage, tile shape) for one precision and theory.
+template <typename T, int TH, int TZ, int TY, class Fn>
+hipError_t visit_stage(int stage, Fn&& fn) {
+  switch (stage) {
+    case 0: return fn.template operator()<0, TZ, TY>();
+    case 1: return fn.template operator()<1, TZ, TY>();
+    case 2: return fn.template operator()<2, TZ, TY>();
+    case 3: return fn.template operator()<3, TZ, TY>();
+    case 4: return fn.template operator()<4, TZ, TY>();
+  }
+  return hipErrorInvalidValue;
+}
+template <typename T, int TH, class Fn>
+hipError_t visit_shape(int stage, const LaunchShape& s, Fn&& fn) {
+  if (s.tz == 64 && s.ty == 4) return visit_stage<T, TH, 64, 4>(stage, fn);
+  if (s.tz == 32 && s.ty == 8) return visit_stage<T, TH, 32, 8>(stage, fn);
+  if (s.tz == 16 && s.ty == 16) return visit_stage<T, TH, 16, 16>(stage, fn);
+  return hipErrorInvalidValue;
+}
+
+template <typename T, int TH>
+struct StageLaunchFn {
+  const StageArgs<T>& a;
+  const LaunchShape& s;
+  hipStream_t st;
+  template <int STAGE, int TZ, int TY>
+  hipError_t operator()() {
+    StageArgs<T> b = a;
+    b.xchunk = s.xchunk;
+    b.nbz = (a.Z + TZ - 1) / TZ;
+    b.nby = (a.Y + TY - 1) / TY;
+    b.nbx = (a.X + s.xchunk - 1) / s.xchunk;
+    const int blocks = b.nbz * b.nby * b.nbx;
+    if constexpr (TH == RHO) {  // curl pass of the three isospins first (same stream)
+      hipLaunchKernelGGL((k_rho_curl<T, STAGE, TZ, TY>), dim3(3 * blocks), dim3(TZ * TY), 0, st, b);
+      const hipError_t r = hipGetLastError();
+      if (r != hipSuccess) return r;
+    }
+    hipLaunchKernelGGL((k_stage<T, TH, STAGE, TZ, TY>), dim3(blocks), dim3(TZ * TY), 0, st, b);
+    return hipGetLastError();
+  }
+};
+
+template <typename T, int TH>
+struct StageBatchLaunchFn {
+  const BatchSlot<T>* slots;
+  const int* blk_inst;
+  int parity, grid;
+  hipStream_t st;
+  template <int STAGE, int TZ, int TY>
+  hipError_t operator()() {
+    hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY>), dim3(grid), dim3(TZ * TY), 0, st, slots, blk_inst,
+                       parity);
+    return hipGetLastError();
+  }
+};
+
+template <typename T, int TH>
+struct StageOccupancyFn {
+  int* out;
+  template <int STAGE, int TZ, int TY>
+  hipError_t operator()() {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        out, reinterpret_cast<const void*>(&k_stage<T, TH, STAGE, TZ, TY>), TZ * TY, 0);
+  }
+};
+
+template <typename T, int TH>
+hipError_t stage_launch(int stage, const StageArgs<T>& a, const LaunchShape& s, hipStream_t st) {
+  return visit_shape<T, TH>(stage, s, StageLaunchFn<T, TH>{a, s, st});
+}
+
+template <typename T, int TH>
+hipError_t stage_launch_batch(int stage, const LaunchShape& s, const BatchSlot<T>* slots, const int* blk_inst,
+                              int parity, int grid, hipStream_t st) {
+  return visit_shape<T, TH>(stage, s, StageBatchLaunchFn<T, TH>{slots, blk_inst, parity, grid, st});
+}
+
+template <typename T, int TH>
+hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks) {
+  return visit_shape<T, TH>(stage, s, StageOccupancyFn<T, TH>{blocks});
+}
+
+// One translation unit per (precision, theory) instantiates these.
+#define SKX_INSTANTIATE_STAGES(T, TH)                                                                          \
+  template hipError_t stage_launch<T, TH>(int, const StageArgs<T>&, const LaunchShape&, hipStream_t);          \
+  template hipError_t stage_launch_batch<T, TH>(int, const LaunchShape&, const BatchSlot<T>*, const int*, int, \
+                                                int, hipStream_t);                                             \
+  template hipError_t stage_occupancy<T, TH>(int, const LaunchShape&, int*);
+// rho runs solo engines only (no batched launches)
+#define SKX_INSTANTIATE_STAGES_SOLO(T, TH)                                                            \
+  template hipError_t stage_launch<T, TH>(int, const StageArgs<T>&, const LaunchShape&, hipStream_t); \
+  template hipError_t stage_occupancy<T, TH>(int, const LaunchShape&, int*);
+
+}  // namespace skx

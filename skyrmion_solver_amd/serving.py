@@ -1,0 +1,180 @@
+"""Continuous-batching minimisation server on one GPU.
+
+Requests (one lattice each: theory, packed parameters, initial field,
+tolerance) arrive over time.  The server
+
+* admits queued requests in arrival order while their device footprint
+  (``skx_footprint_bytes``) fits the HBM budget and their theory's batch has
+  a free slot and enough free blocks (head-of-line order is kept: a request
+  that does not fit blocks the ones behind it);
+* keeps every admitted lattice resident on the device until it converges
+  (stage-4 max|g| < tol) or hits its step limit;
+* advances all resident lattices in chunks of steps, one batched launch per
+  RK4 stage per theory (``skx_batch_launch``), the batches of different
+  theories on their own streams so their work overlaps;
+* admits newly arrived requests between chunks, so requests that arrive
+  while others run join them on the device (continuous batching);
+* reports time-to-converged (finish - arrival) p50 / p99 and throughput.
+
+A lattice's result does not depend on what it shares the device with: each
+one owns its buffers, reductions and status, and the batched kernels compute
+every site exactly as a solo run does.
+"""
+
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from skyrmion_solver_amd import _engine
+
+THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
+              "rho": _engine.THEORY_RHO}
+
+
+@dataclass
+class Request:
+    rid: int
+    theory: str
+    p_i: np.ndarray
+    p_f: np.ndarray
+    F0: np.ndarray
+    tol: float = 1.0e-4
+    max_steps: int = 20000
+    arrival: float = 0.0
+    V0: np.ndarray | None = None
+    # filled in by the server
+    admitted: float | None = None
+    finished: float | None = None
+    status: dict = field(default_factory=dict)
+    F: np.ndarray | None = None
+    V: np.ndarray | None = None
+
+    @property
+    def sites(self) -> int:
+        return int(self.p_i[6])
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class Server:
+    """Admission by HBM footprint + continuous batching over ``skx_batch`` handles."""
+
+    def __init__(self, precision: int = _engine.FP32, hbm_budget_bytes: int = 64 << 30, chunk_steps: int = 32,
+                 max_slots: int = 64, max_blocks: int = 32768, tile_z: int = 32, device: int = 0,
+                 keep_fields: bool = True, batch_factory=None, footprint_fn=None, clock=time.perf_counter):
+        self.precision = precision
+        self.budget = int(hbm_budget_bytes)
+        self.chunk = int(chunk_steps)
+        self.max_slots, self.max_blocks, self.tile_z, self.device = max_slots, max_blocks, tile_z, device
+        self.keep_fields = keep_fields
+        self._batch_factory = batch_factory or (lambda theory: _engine.Batch(
+            THEORY_IDS[theory], precision, max_slots=max_slots, max_blocks=max_blocks, tile_z=tile_z, device=device))
+        self._footprint = footprint_fn or (lambda r: _engine.footprint_bytes(
+            THEORY_IDS[r.theory], precision, r.p_i, r.p_f, tile_z=tile_z))
+        self.clock = clock
+        self.t0 = clock()
+        self.batches: dict = {}
+        self.queue: deque = deque()
+        self.running: dict = {}  # (theory, slot) -> (request, footprint)
+        self.used = 0
+        self.done: list = []
+        self.site_updates = 0
+        self.peak_used = 0
+
+    def now(self) -> float:
+        return self.clock() - self.t0
+
+    def submit(self, req: Request):
+        self.queue.append(req)
+
+    def _batch(self, theory: str):
+        if theory not in self.batches:
+            self.batches[theory] = self._batch_factory(theory)
+        return self.batches[theory]
+
+    def admit(self):
+        while self.queue:
+            req = self.queue[0]
+            fp = self._footprint(req)
+            if fp > self.budget:
+                raise ValueError(f"request {req.rid} needs {fp} bytes, more than the whole budget")
+            if self.used + fp > self.budget:
+                return
+            try:
+                slot = self._batch(req.theory).add(req.p_i, req.p_f, req.F0, req.V0, max_steps=req.max_steps,
+                                                   tol=req.tol, device=self.device)
+            except _engine.SkxError as e:
+                if e.code == -5:  # SKX_EBUSY: no slot / blocks free in this batch now
+                    return
+                raise
+            self.queue.popleft()
+            req.admitted = self.now()
+            self.running[(req.theory, slot)] = (req, fp)
+            self.used += fp
+            self.peak_used = max(self.peak_used, self.used)
+
+    def step(self):
+        """Admit, advance every resident lattice by one chunk, retire the finished ones."""
+        self.admit()
+        active = sorted({t for (t, _) in self.running})
+        for t in active:  # enqueue every theory's chunk first: their streams overlap
+            self.batches[t].launch(self.chunk)
+        for t in active:
+            self.batches[t].wait()
+        for t in active:
+            b = self.batches[t]
+            for st in b.poll():
+                key = (t, st["slot"])
+                if key not in self.running or not st["done"]:
+                    continue
+                req, fp = self.running.pop(key)
+                final, F, V = b.remove(st["slot"], want_fields=self.keep_fields)
+                req.finished = self.now()
+                req.status = final
+                req.F, req.V = F, V
+                self.used -= fp
+                self.site_updates += req.sites * int(final["steps"])
+                self.done.append(req)
+
+    def idle(self) -> bool:
+        return not self.queue and not self.running
+
+    def serve(self, requests, poll_sleep: float = 1.0e-4):
+        """Replay `requests` (their `arrival` in seconds from now) until all have finished."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        self.t0 = self.clock()
+        while pending or not self.idle():
+            now = self.now()
+            while pending and pending[0].arrival <= now:
+                self.submit(pending.popleft())
+            if self.idle():
+                time.sleep(min(poll_sleep, max(0.0, pending[0].arrival - now)))
+                continue
+            self.step()
+        return self.report()
+
+    def report(self) -> dict:
+        lat = np.array([r.latency for r in self.done]) if self.done else np.zeros(1)
+        span = max((r.finished for r in self.done), default=0.0) - min((r.arrival for r in self.done), default=0.0)
+        return dict(requests=len(self.done), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                    mean_s=float(lat.mean()), makespan_s=float(span),
+                    site_updates_per_s=self.site_updates / span if span > 0 else 0.0,
+                    converged=sum(int(r.status.get("converged", 0)) for r in self.done),
+                    peak_hbm_bytes=self.peak_used)
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
+        self.batches.clear()
+
+
+def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
+    """Arrival times (s) of a Poisson stream of `rate` requests per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

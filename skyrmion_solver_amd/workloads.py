@@ -1,0 +1,94 @@
+"""Synthetic benchmark workloads (seeded host initial data + packed parameters).
+
+Each builder returns ``(theory_module, resolved_params, p_i, p_f, F0)``.  The
+time step is the reference's example setting ``courant = 0.2``
+(``examples/nuclear_skyrme_gl.py:13``): the larger dt = 0.5 dx these
+workloads were specified with is beyond the RK4 stability limit of the
+reference's own scheme (the run goes non-finite within tens of steps).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from skyrmion_solver_amd import initial_data
+from skyrmion_solver_amd.theories import load_theory
+
+COURANT = 0.2
+
+
+def _lattice(L, dx):
+    return dict(xlen=L, ylen=L, zlen=L, lsx=dx, lsy=dx, lsz=dx, xsize=dx * (L - 1), ysize=dx * (L - 1),
+                zsize=dx * (L - 1), courant=COURANT)
+
+
+def c1_hedgehog(seed: int = 0, L: int = 32):
+    """Pion Skyrme, B=1 hedgehog f = pi e^{-r}, 32^3, dx = 0.2, m = 0, random SU(2) isorotation, centre jitter 0.1."""
+    th = load_theory("Nuclear Skyrme model")
+    rp = th.params.default_params(**_lattice(L, 0.2), mpi=0.0, kappa=1.0).resolved()
+    p_i, p_f = th.params.pack_device_params(rp)
+    F = initial_data.build_field(rp, 4, dict(mode="hedgehog", profile=("exp", 1.0), jitter=0.1, isorotation="haar",
+                                            seed=seed, vacuum_band=True))
+    return th, rp, p_i, p_f, F
+
+
+def c2_b4(seed: int = 0, L: int = 128):
+    """Pion Skyrme, B=4 rational map f = pi e^{-r/1.5}, 128^3, dx = 0.15, m = 1, N(0, 1e-3) noise."""
+    th = load_theory("Nuclear Skyrme model")
+    rp = th.params.default_params(**_lattice(L, 0.15), mpi=1.0, kappa=1.0, baryon_number=4.0,
+                                  ansatz="rational_map").resolved()
+    p_i, p_f = th.params.pack_device_params(rp)
+    F = initial_data.build_field(rp, 4, dict(mode="rational_map", baryon_number=4, profile=("exp", 1.5),
+                                            noise=1.0e-3, seed=seed, vacuum_band=True))
+    return th, rp, p_i, p_f, F
+
+
+def c3_b8(theory: str = "omega", seed: int = 0, L: int = 160):
+    """Omega or Coulomb Skyrme, B=8 = product of two B=4 rational maps at -+1.5 along x, 160^3, dx = 0.15.
+
+    omega: c_omega = 0.5 (omega mass 1 in kernel units), pion mass the
+    derived default m_pi / m_omega; omega starts at -c_omega * CS.
+    Coulomb: kappa = 0.1, pion mass the derived default; V starts at 0.
+    """
+    name = {"omega": "Omega-Skyrme model", "coulomb": "Coulomb-Skyrme model"}[theory]
+    th = load_theory(name)
+    extra = dict(c_omega=0.5) if theory == "omega" else dict(kappa=0.1)
+    rp = th.params.default_params(**_lattice(L, 0.15), baryon_number=8.0, **extra).resolved()
+    p_i, p_f = th.params.pack_device_params(rp)
+    lump = dict(mode="rational_map", baryon_number=4, profile=("exp", 1.5))
+    F = initial_data.build_field(rp, 5, dict(mode="product", seed=seed, vacuum_band=True,
+                                            lumps=[dict(lump, center=(-1.5, 0.0, 0.0)),
+                                                   dict(lump, center=(1.5, 0.0, 0.0))]))
+    return th, rp, p_i, p_f, F
+
+
+def c4_instances(n: int = 48, seed: int = 0, sizes=(48, 64, 96, 128), dx: float = 0.15):
+    """Independent pion lattices: L from `sizes`, B = 1..6 rational maps with random space and
+    isospin orientations, m ~ U[0, 1.5]."""
+    rng = np.random.default_rng(seed)
+    th = load_theory("Nuclear Skyrme model")
+    out = []
+    for k in range(n):
+        L = int(rng.choice(sizes))
+        B = int(rng.integers(1, 7))
+        m = float(rng.uniform(0.0, 1.5))
+        rp = th.params.default_params(**_lattice(L, dx), mpi=m, kappa=1.0, baryon_number=float(B)).resolved()
+        p_i, p_f = th.params.pack_device_params(rp)
+        F = initial_data.build_field(rp, 4, dict(mode="rational_map", baryon_number=B, profile=("exp", 1.0 + 0.15 * B),
+                                                isorotation="haar", space_rotation="haar",
+                                                seed=int(rng.integers(1 << 30)), vacuum_band=True))
+        out.append(dict(L=L, B=B, m=m, rp=rp, p_i=p_i, p_f=p_f, F=F))
+    return out
+
+
+def set_omega_from_chern_simons(engine, rp):
+    """omega = -c_omega * CS of the uploaded pion components (``create_omega_field_kernel``,
+    ``theories/omega_skyrme/initial_config.py:1495-1532``), computed with the engine's baryon density."""
+    from skyrmion_solver_amd import _engine
+
+    F = engine.download(_engine.FIELD)
+    cs_vol = engine.density(_engine.DENSITY_BARYON)
+    N = cs_vol.size
+    F[4 * N:5 * N] = -float(rp.c_omega) * (cs_vol / float(rp.grid_volume))
+    engine.upload(_engine.FIELD, F)
+    return F

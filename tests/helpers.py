@@ -23,7 +23,8 @@ def make_params(theory: str, X: int, Y: int, Z: int, **kw):
     if theory == "coulomb":
         base.update(kappa=0.3, mass=0.6)
     if theory == "rho":
-        base.update(mpi=0.5, mrho=1.2, c_alpha=0.3)
+        # the rho Laplacian enters with weight 8 (pion: 2): RK4 needs the smaller step
+        base.update(mpi=0.5, mrho=1.2, c_alpha=0.3, time_step=0.02)
     base.update(kw)
     p = mod.params.default_params(**base)
     rp = p.resolved()

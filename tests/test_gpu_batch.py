@@ -1,0 +1,104 @@
+"""GPU: batched launches give every lattice exactly its solo result.
+
+A batch runs lattices of different sizes and parameters with one launch
+per RK4 stage; each lattice must end bit-identical to the same lattice run
+alone (the solo engine uses the same one-chunk x decomposition, so even the
+FP64 reductions are summed in the same order), and lattices that join while
+others are half-way through must not perturb them.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import make_params, smooth_field
+from skyrmion_solver_amd import _engine
+
+pytestmark = pytest.mark.gpu
+
+TID = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB}
+
+
+def lattices(theory, shapes, seed0=0):
+    out = []
+    for k, sh in enumerate(shapes):
+        kw = {"mpi": 0.5 + 0.2 * k} if theory == "pion" else {"mass": 0.4 + 0.1 * k}
+        mod, rp, p_i, p_f = make_params(theory, *sh, **kw)
+        F = smooth_field(rp, int(p_i[5]), seed=seed0 + k, normalise=True)
+        out.append((p_i, p_f, F))
+    return out
+
+
+def solo(theory, prec, p_i, p_f, F, steps, tol):
+    with _engine.Engine(TID[theory], prec, p_i, p_f, tile_z=32, x_chunk=int(p_i[0])) as e:
+        e.upload(_engine.FIELD, F)
+        st = e.anf_run(steps, tol, 16)
+        return st, e.download(_engine.FIELD), e.download(_engine.VELOCITY)
+
+
+@pytest.mark.parametrize("theory,prec", [("pion", _engine.FP64), ("pion", _engine.FP32),
+                                         ("coulomb", _engine.FP32), ("omega", _engine.FP64)])
+def test_batch_equals_solo_bitwise(theory, prec):
+    print(f"[case] batch theory={theory} prec={prec}", flush=True)
+    lat = lattices(theory, [(20, 18, 16), (16, 16, 16), (24, 21, 34)])
+    steps = [30, 24, 30]
+    with _engine.Batch(TID[theory], prec, max_slots=8, max_blocks=2048) as b:
+        slots = [b.add(p_i, p_f, F, max_steps=s) for (p_i, p_f, F), s in zip(lat, steps)]
+        for _ in range(4):
+            b.run(8)
+        res = [b.remove(s) for s in slots]
+    for (p_i, p_f, F), s, (st, Fb, Vb) in zip(lat, steps, res):
+        ref, Fs, Vs = solo(theory, prec, p_i, p_f, F, s, -1.0)
+        assert st["steps"] == ref.steps == s
+        assert st["energy"] == ref.energy
+        assert st["err_abs"] == ref.err_abs
+        assert np.array_equal(Fb, Fs)
+        assert np.array_equal(Vb, Vs)
+
+
+def test_late_joiner_does_not_perturb_running_lattices():
+    print("[case] batch late joiner", flush=True)
+    lat = lattices("pion", [(18, 18, 18), (22, 16, 20), (16, 20, 18)], seed0=10)
+    with _engine.Batch(TID["pion"], _engine.FP32, max_slots=4, max_blocks=1024) as b:
+        s0 = b.add(*lat[0], max_steps=40)
+        b.run(10)
+        s1 = b.add(*lat[1], max_steps=40)
+        b.run(10)
+        s2 = b.add(*lat[2], max_steps=40)
+        for _ in range(4):
+            b.run(10)
+        out = {s: b.remove(s) for s in (s0, s1, s2)}
+    for (p_i, p_f, F), s in zip(lat, (s0, s1, s2)):
+        ref, Fs, _ = solo("pion", _engine.FP32, p_i, p_f, F, 40, -1.0)
+        st, Fb, _ = out[s]
+        assert st["steps"] == 40
+        assert st["energy"] == ref.energy
+        assert np.array_equal(Fb, Fs)
+
+
+def test_batch_convergence_stops_each_lattice():
+    print("[case] batch convergence", flush=True)
+    lat = lattices("pion", [(16, 16, 16), (18, 18, 18)], seed0=20)
+    tol = 5.0
+    with _engine.Batch(TID["pion"], _engine.FP64, max_slots=4, max_blocks=1024) as b:
+        slots = [b.add(p_i, p_f, F, max_steps=2000, tol=tol) for p_i, p_f, F in lat]
+        for _ in range(20):
+            b.run(16)
+            if all(s["done"] for s in b.poll()):
+                break
+        res = [b.remove(s) for s in slots]
+    for (p_i, p_f, F), (st, Fb, _) in zip(lat, res):
+        ref, Fs, _ = solo("pion", _engine.FP64, p_i, p_f, F, 2000, tol)
+        assert st["converged"] == 1 and ref.converged == 1
+        assert st["steps"] == ref.steps
+        assert np.array_equal(Fb, Fs)
+
+
+def test_batch_rejects_when_full():
+    lat = lattices("pion", [(16, 16, 16)])
+    with _engine.Batch(TID["pion"], _engine.FP32, max_slots=1, max_blocks=64) as b:
+        b.add(*lat[0])
+        with pytest.raises(_engine.SkxError) as e:
+            b.add(*lat[0])
+        assert e.value.code == -5

@@ -18,7 +18,8 @@ from skyrmion_solver_amd import _engine
 
 pytestmark = pytest.mark.gpu
 
-THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB}
+THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
+              "rho": _engine.THEORY_RHO}
 SHAPES = [(20, 18, 16), (9, 7, 13), (24, 40, 70)]
 
 
@@ -27,7 +28,7 @@ def _engine_for(theory, rp, p_i, p_f, prec=_engine.FP64, **kw):
     return _engine.Engine(THEORY_IDS[theory], prec, p_i, p_f, **kw)
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_gradient_probe_fp64(theory, shape):
     mod, rp, p_i, p_f = make_params(theory, *shape)
@@ -43,19 +44,21 @@ def test_gradient_probe_fp64(theory, shape):
     assert np.max(np.abs(eg - EG_ref.ravel())) <= 1e-10 * np.max(np.abs(EG_ref))
 
 
+@pytest.mark.parametrize("theory", ["pion", "rho"])
 @pytest.mark.parametrize("tile_z", [16, 32, 64])
-def test_gradient_probe_tiles(tile_z):
-    mod, rp, p_i, p_f = make_params("pion", 19, 21, 67)
-    T = smooth_field(rp, 4, seed=7)
-    st = OI.OracleState("pion", p_i, p_f, T)
+def test_gradient_probe_tiles(tile_z, theory):
+    mod, rp, p_i, p_f = make_params(theory, 19, 21, 67)
+    nf = int(p_i[5])
+    T = smooth_field(rp, nf, seed=7)
+    st = OI.OracleState(theory, p_i, p_f, T)
     EG_ref, _ = OI.gradient(st, st.F)
     for xc in (3, 8, 19):
-        with _engine_for("pion", rp, p_i, p_f, tile_z=tile_z, x_chunk=xc) as eng:
+        with _engine_for(theory, rp, p_i, p_f, tile_z=tile_z, x_chunk=xc) as eng:
             eg, _ = eng.eval_gradient(T)
         assert rel_l2(eg, EG_ref) < 1e-12, (tile_z, xc)
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
 def test_energy_baryon_fp64(theory):
     mod, rp, p_i, p_f = make_params(theory, 22, 20, 18)
     nf = int(p_i[5])
@@ -75,7 +78,7 @@ def test_energy_baryon_fp64(theory):
     assert rel_l2(db, OI.baryon_density(st)) < 1e-12
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
 def test_anf_steps_fp64(theory):
     mod, rp, p_i, p_f = make_params(theory, 20, 18, 16)
     nf = int(p_i[5])
@@ -120,7 +123,7 @@ def test_arrest_applied_like_reference():
     print("arrests:", arrests)
 
 
-@pytest.mark.parametrize("theory", ["pion", "coulomb"])
+@pytest.mark.parametrize("theory", ["pion", "coulomb", "rho"])
 def test_run_equals_steps_bitwise(theory):
     """The device-resident loop (fused energy, deferred arrest, hipGraph) equals repeated step()."""
     mod, rp, p_i, p_f = make_params(theory, 18, 17, 20, time_step=0.04)
@@ -154,7 +157,7 @@ def test_run_no_graph_equals_graph():
         assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
 def test_fp32_form_against_fp64_oracle(theory):
     """FP32 storage/stencils: after 10 steps, |dE|/E <= 1e-5, |dB| <= 1e-4, field rel-L2 <= 1e-5."""
     mod, rp, p_i, p_f = make_params(theory, 20, 18, 16)

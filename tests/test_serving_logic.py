@@ -1,0 +1,118 @@
+"""Host logic of the serving loop (admission by footprint, FIFO order,
+continuous batching, latency accounting) against a fake batch; no GPU."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from skyrmion_solver_amd import _engine
+from skyrmion_solver_amd.serving import Request, Server, poisson_arrivals
+
+
+class FakeBatch:
+    """Each lattice finishes after `p_i[0]` chunks; at most `slots` resident."""
+
+    def __init__(self, theory, slots, log):
+        self.theory, self.slots, self.log = theory, slots, log
+        self.lat = {}
+
+    def add(self, p_i, p_f, F0, V0, max_steps, tol, device):
+        if len(self.lat) >= self.slots:
+            raise _engine.SkxError(-5, "skx_batch_add", "busy")
+        s = min(set(range(self.slots)) - set(self.lat))
+        self.lat[s] = dict(left=int(p_i[0]), steps=0)
+        self.log.append(("add", self.theory, int(p_i[1])))
+        return s
+
+    def launch(self, steps):
+        for v in self.lat.values():
+            if v["left"] > 0:
+                v["left"] -= 1
+                v["steps"] += steps
+
+    def wait(self):
+        pass
+
+    def poll(self):
+        return [dict(slot=s, done=int(v["left"] == 0), steps=v["steps"]) for s, v in self.lat.items()]
+
+    def remove(self, slot, want_fields=True):
+        v = self.lat.pop(slot)
+        return dict(steps=v["steps"], converged=1, energy=1.0), None, None
+
+    def close(self):
+        pass
+
+
+def make_requests(n, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        chunks = int(rng.integers(1, 6))
+        p_i = np.array([chunks, k, 1, 2, 3, 4, 1000, 4000], dtype=np.int32)
+        out.append(Request(rid=k, theory=["pion", "omega"][k % 2], p_i=p_i, p_f=np.zeros(10), F0=np.zeros(4),
+                           arrival=0.0))
+    return out
+
+
+def test_admission_respects_budget_and_fifo():
+    log = []
+    fp = {k: 100 + 10 * (k % 3) for k in range(12)}
+    srv = Server(hbm_budget_bytes=350, chunk_steps=8, batch_factory=lambda t: FakeBatch(t, 8, log),
+                 footprint_fn=lambda r: fp[r.rid])
+    reqs = make_requests(12)
+    for r in reqs:
+        srv.submit(r)
+    order = []
+    while not srv.idle():
+        srv.step()
+        assert srv.used <= 350
+        order = [e[2] for e in log]
+    assert order == sorted(order)  # admitted in arrival order
+    assert len(srv.done) == 12 and srv.used == 0
+    assert srv.peak_used <= 350
+
+
+def test_busy_batch_defers_admission():
+    log = []
+    srv = Server(hbm_budget_bytes=10 ** 9, chunk_steps=4, batch_factory=lambda t: FakeBatch(t, 1, log),
+                 footprint_fn=lambda r: 1)
+    reqs = [r for r in make_requests(6) if r.theory == "pion"]
+    for r in reqs:
+        srv.submit(r)
+    srv.step()
+    assert len(srv.running) == 1 and len(srv.queue) == len(reqs) - 1
+    while not srv.idle():
+        srv.step()
+    assert [r.rid for r in srv.done] == [r.rid for r in reqs]
+
+
+def test_late_arrivals_join_running_work():
+    log = []
+    t = [0.0]
+
+    def clock():  # fake time: every read advances 2 ms
+        t[0] += 0.002
+        return t[0]
+
+    srv = Server(hbm_budget_bytes=10 ** 9, chunk_steps=4, batch_factory=lambda th: FakeBatch(th, 8, log),
+                 footprint_fn=lambda r: 1, clock=clock)
+    reqs = make_requests(8, seed=3)
+    arr = poisson_arrivals(8, rate=50.0, seed=1)
+    for r, a in zip(reqs, arr):
+        r.arrival = float(a)
+    rep = srv.serve(reqs, poll_sleep=0.0)
+    assert rep["requests"] == 8
+    overlapped = any(a.admitted < b.finished and b.admitted < a.finished
+                     for a in srv.done for b in srv.done if a is not b)
+    assert overlapped
+    assert all(r.latency >= 0 for r in srv.done)
+    assert rep["p99_s"] >= rep["p50_s"]
+
+
+def test_request_larger_than_budget_is_rejected():
+    srv = Server(hbm_budget_bytes=10, batch_factory=lambda t: FakeBatch(t, 4, []), footprint_fn=lambda r: 11)
+    srv.submit(make_requests(1)[0])
+    with pytest.raises(ValueError):
+        srv.step()
