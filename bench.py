@@ -51,7 +51,11 @@ def workload(L: int = 128, dx: float = 0.15, seed: int = 0):
     return rp, p_i, p_f, F
 
 
-def time_form(prec, p_i, p_f, F, steps, warmup, chunk):
+def time_form(prec, p_i, p_f, F, steps, warmup, chunk, trace=True):
+    """Timed device-resident run (hipGraph chunks of `chunk` steps).  With
+    `trace`, the same number of steps is then run again from the same field
+    with every stage kernel bracketed by HIP events on the engine's stream
+    (direct launches), giving per-kernel average durations of this workload."""
     eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
     eng.upload(_engine.FIELD, F)
     eng.anf_run(warmup, -1.0, chunk)
@@ -62,23 +66,35 @@ def time_form(prec, p_i, p_f, F, steps, warmup, chunk):
     dt = time.perf_counter() - t0
     out = dict(seconds=dt, steps=st.steps, energy=st.energy, err_abs=st.err_abs, baryon=st.baryon)
     eng.close()
+    if trace:
+        eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f, flags=_engine.FLAG_TRACE)
+        eng.upload(_engine.FIELD, F)
+        eng.anf_run(warmup, -1.0, chunk)
+        eng.trace_reset()
+        eng.anf_run(steps, -1.0, chunk)
+        ms, cnt = eng.trace_read()
+        out["kernels"] = {STAGE_NAMES[k]: dict(avg_us=1000.0 * ms[k] / cnt[k], launches=int(cnt[k]))
+                          for k in range(5) if cnt[k]}
+        eng.close()
     return out
 
 
-def trace_form(prec, p_i, p_f, F, steps):
-    """Per-kernel durations from HIP events on the engine's stream (direct launches)."""
-    eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f, flags=_engine.FLAG_TRACE)
-    eng.upload(_engine.FIELD, F)
-    eng.anf_run(3, -1.0, 3)
-    eng.trace_reset()
-    eng.anf_run(steps, -1.0, steps)
-    ms, cnt = eng.trace_read()
-    eng.close()
-    return {STAGE_NAMES[k]: dict(avg_us=1000.0 * ms[k] / cnt[k], launches=int(cnt[k])) for k in range(5) if cnt[k]}
+def profiled_traffic(kernel: str, form: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"stage_kernels_pion_128_b4_{form}.md")
+    key = f"traffic_bytes_per_launch {kernel} {form}:"
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split(":")[1])
+    return None
 
 
 def solve_latency(prec, p_i, p_f, seeds, tol, max_steps):
-    lat, steps = [], []
+    """Time-to-converged (max|g| < tol) from each seed's initial field, with the converged E and B."""
+    lat, steps, energy, baryon = [], [], [], []
     for s in seeds:
         _, _, _, F = workload(seed=s)
         eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
@@ -89,10 +105,12 @@ def solve_latency(prec, p_i, p_f, seeds, tol, max_steps):
         eng.sync()
         lat.append(time.perf_counter() - t0)
         steps.append(st.steps)
+        energy.append(st.energy)
+        baryon.append(st.baryon)
         eng.close()
     lat = np.array(lat)
     return dict(tol=tol, seeds=len(seeds), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                steps=steps)
+                steps=steps, energy=energy, baryon=baryon)
 
 
 def cpu_baseline(p_i, p_f, F, steps: int = 1):
@@ -111,6 +129,64 @@ def cpu_baseline(p_i, p_f, F, steps: int = 1):
                        f"{dt:.1f} s")
 
 
+def batch_sample(n: int, steps: int):
+    """Batched form: `n` independent pion lattices (L uniform in {48, 64, 96, 128}^3, B = 1..6 rational maps
+    with random orientations, m ~ U[0, 1.5]) advanced together, FP32.  The batch's tile and x-chunk rule
+    come from the autotuner (timed on one lattice per size class); the comparison is the same lattices run
+    one after another as solo engines with the engine's default launch shape."""
+    from skyrmion_solver_amd import autotune, workloads
+
+    inst = workloads.c4_instances(n=n, seed=0)
+    per_class = {}
+    for d in inst:
+        per_class.setdefault(d["L"], d)
+    tuned = autotune.tune_batch(_engine.THEORY_PION, _engine.FP32,
+                                [(d["p_i"], d["p_f"]) for d in per_class.values()], steps=8, use_cache=False)
+    tz, tgt = tuned["tile_z"], tuned["target"]
+    dims = [(d["L"],) * 3 for d in inst]
+    blocks = autotune.batch_blocks(dims, tz, tgt)
+    sites = sum(d["L"] ** 3 for d in inst)
+    with _engine.Batch(_engine.THEORY_PION, _engine.FP32, max_slots=len(inst), max_blocks=blocks, tile_z=tz) as b:
+        for d in inst:
+            b.add(d["p_i"], d["p_f"], d["F"], max_steps=1 << 30, x_chunk=autotune.chunk_rule(d["L"], tgt))
+        b.run(4)
+        t0 = time.perf_counter()
+        b.run(steps)
+        dt = time.perf_counter() - t0
+    # solo baseline: one engine per size class, scaled by the class count
+    solo_s = 0.0
+    for L, d in per_class.items():
+        count = sum(1 for e in inst if e["L"] == L)
+        eng = _engine.Engine(_engine.THEORY_PION, _engine.FP32, d["p_i"], d["p_f"])
+        eng.upload(_engine.FIELD, d["F"])
+        eng.anf_run(4, -1.0, 4)
+        st = eng.anf_run(steps, -1.0, steps)
+        eng.close()
+        solo_s += count * st.seconds
+    return dict(instances=len(inst), sites=sites, steps=steps, site_updates_per_s=sites * steps / dt,
+                solo_sequential_site_updates_per_s=sites * steps / solo_s, tile_z=tz, x_chunk_target=tgt,
+                blocks=blocks)
+
+
+def serving_sample(n: int, rate: float, sizes, max_steps: int):
+    """Serving form: a Poisson stream of `n` minimisation requests at `rate` per second (pion 60 %, omega
+    25 %, rho 15 %; L from `sizes`; B = 1..8) served to max|g| < 1e-4 by continuous batching, FP32 and
+    FP64 forms; time-to-converged p50 / p99 and throughput."""
+    from skyrmion_solver_amd import serving, workloads
+
+    out = {}
+    for name, prec in (("fp32", _engine.FP32), ("fp64", _engine.FP64)):
+        reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps)
+        srv = serving.Server(precision=prec, hbm_budget_bytes=64 << 30, chunk_steps=32, max_slots=64,
+                             max_blocks=32768)
+        rep = srv.serve(reqs)
+        srv.close()
+        rep["by_theory"] = {t: sum(1 for r in reqs if r.theory == t) for t in ("pion", "omega", "rho")}
+        out[name] = rep
+    out.update(requests=n, rate_per_s=rate, sizes=list(sizes), max_steps=max_steps)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,12 +194,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--chunk", type=int, default=100,
                     help="steps per captured graph; --steps should be a multiple so no launch is an early exit")
-    ap.add_argument("--trace-steps", type=int, default=20)
+    ap.add_argument("--no-trace", action="store_true", help="time without the per-kernel HIP events")
     ap.add_argument("--solve-seeds", type=int, default=3)
     ap.add_argument("--solve-max-steps", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1.0e-4)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-n", type=int, default=48, help="lattices of the batched sample (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=40)
+    ap.add_argument("--serve-n", type=int, default=24, help="requests of the serving sample (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-sizes", type=int, nargs="+", default=[48, 64, 96])
+    ap.add_argument("--serve-max-steps", type=int, default=6000)
     ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
                     help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
     args = ap.parse_args()
@@ -140,33 +222,34 @@ def main():
     nf = 4
     if args.only in ("fp32", "fp64"):
         prec = _engine.FP32 if args.only == "fp32" else _engine.FP64
-        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk)
+        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk, trace=not args.no_trace)
         print(f"[bench] {args.only}: {N * r['steps'] / r['seconds']:.4g} site-updates/s", file=sys.stderr, flush=True)
         return
     copy_gbs, _ = _engine.stream_copy_bandwidth(1 << 30, 20)
     res = {}
     for name, prec in (("fp64", _engine.FP64), ("fp32", _engine.FP32)):
-        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk)
+        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk, trace=not args.no_trace)
         r["site_updates_per_s"] = N * r["steps"] / r["seconds"]
         r["ms_per_step"] = 1000.0 * r["seconds"] / r["steps"]
         s = 8 if prec == _engine.FP64 else 4
         r["algorithmic_GBs"] = r["site_updates_per_s"] * 16 * nf * s / 1e9
-        r["kernels"] = trace_form(prec, p_i, p_f, F, args.trace_steps)
         res[name] = r
         print(f"[bench] {name}: {r['site_updates_per_s']:.4g} site-updates/s, {r['ms_per_step']:.4f} ms/step, "
               f"{r['algorithmic_GBs']:.1f} GB/s algorithmic", file=sys.stderr, flush=True)
 
     head = res["fp32"]
-    # dominant kernel of the headline form: largest total time per step
-    kern = head["kernels"]
-    dom = max(kern, key=lambda k: kern[k]["avg_us"] * (1 if k != "energy" else 0))
-    stage = [k for k, v in STAGE_NAMES.items() if v == dom][0]
-    bytes_per_launch = STAGE_VECTORS[stage] * nf * 4 * N
-    achieved = bytes_per_launch / (kern[dom]["avg_us"] * 1e-6) / 1e9
-    roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None,
-                    bytes_per_launch=bytes_per_launch, measured_copy_GBs=copy_gbs,
-                    frac_of_copy=achieved / copy_gbs)
+    roofline = None
+    if "kernels" in head:
+        # dominant kernel of the headline form: largest time per step
+        kern = head["kernels"]
+        dom = max(kern, key=lambda k: kern[k]["avg_us"] * (1 if k != "energy" else 0))
+        stage = [k for k, v in STAGE_NAMES.items() if v == dom][0]
+        bytes_per_launch = STAGE_VECTORS[stage] * nf * 4 * N
+        achieved = bytes_per_launch / (kern[dom]["avg_us"] * 1e-6) / 1e9
+        roofline = dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=achieved / HBM_PEAK_GBS, traffic=profiled_traffic(dom, "fp32"),
+                        bytes_per_launch=bytes_per_launch, measured_copy_GBs=copy_gbs,
+                        frac_of_copy=achieved / copy_gbs)
 
     latency = {}
     if args.solve_seeds > 0:
@@ -174,6 +257,22 @@ def main():
             latency[name] = solve_latency(prec, p_i, p_f, list(range(args.solve_seeds)), args.tol,
                                           args.solve_max_steps)
             print(f"[bench] solve {name}: {latency[name]}", file=sys.stderr, flush=True)
+
+    accuracy = None
+    if "fp32" in latency and "fp64" in latency:
+        e32, e64 = np.array(latency["fp32"]["energy"]), np.array(latency["fp64"]["energy"])
+        b32, b64 = np.array(latency["fp32"]["baryon"]), np.array(latency["fp64"]["baryon"])
+        accuracy = dict(max_rel_dE=float(np.max(np.abs(e32 - e64) / np.abs(e64))),
+                        max_abs_dB=float(np.max(np.abs(b32 - b64))),
+                        note="FP32 form vs FP64 form at convergence (max|g| < tol), same seeds")
+
+    batch = batch_sample(args.batch_n, args.batch_steps) if args.batch_n > 0 else None
+    if batch:
+        print(f"[bench] batch: {batch}", file=sys.stderr, flush=True)
+    serve = serving_sample(args.serve_n, args.serve_rate, args.serve_sizes, args.serve_max_steps) \
+        if args.serve_n > 0 else None
+    if serve:
+        print(f"[bench] serving: {serve}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -201,6 +300,9 @@ def main():
         "fp32": {"algorithmic_GBs": head["algorithmic_GBs"], "kernels": head["kernels"]},
         "fp32_over_fp64": head["site_updates_per_s"] / res["fp64"]["site_updates_per_s"],
         "solve_latency": latency,
+        "fp32_accuracy": accuracy,
+        "batch": batch,
+        "serving": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

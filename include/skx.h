@@ -61,7 +61,8 @@ enum skx_density_kind {
 
 enum skx_flags {
   SKX_FLAG_NO_GRAPH = 1, /* launch stage kernels directly instead of replaying a captured hipGraph */
-  SKX_FLAG_TRACE = 2     /* time every stage kernel with HIP events */
+  SKX_FLAG_TRACE = 2     /* time every stage kernel with a pair of HIP events on the handle's stream
+                            (runs then launch directly, without graphs) */
 };
 
 /* Lattice + theory description.  p_i / p_f are the reference's packed
@@ -152,7 +153,8 @@ int skx_history_attach(skx_handle h, int32_t capacity);
 int skx_history_read(skx_handle h, skx_step_record* out, int32_t n, int32_t* n_read);
 
 /* Stage-kernel timing (SKX_FLAG_TRACE): per stage kind 0..4 (0 = energy
- * pass, 1..4 = RK4 stages), accumulated milliseconds and launch counts. */
+ * pass, 1..4 = RK4 stages), accumulated milliseconds and launch counts.  For
+ * the rho theory a stage's time covers its curl pass and stage kernel. */
 int skx_trace_read(skx_handle h, double ms[5], int64_t counts[5]);
 int skx_trace_reset(skx_handle h);
 
@@ -185,7 +187,10 @@ typedef struct skx_slot_status {
 int skx_batch_create(const skx_batch_desc* desc, skx_batch_handle* out);
 int skx_batch_destroy(skx_batch_handle b);
 /* Add a lattice (F0 / V0 host FP64, V0 may be NULL); it runs until its
- * stage-4 max|g| < tol or max_steps.  SKX_EBUSY when no slot / blocks are free. */
+ * stage-4 max|g| < tol or max_steps.  lattice->x_chunk = planes per block
+ * along x (0: the whole lattice, one block per (y, z) tile); lattice->tile_z
+ * is ignored (the batch's tile applies).  SKX_EBUSY when no slot / blocks are
+ * free. */
 int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0, const double* V0, int32_t max_steps,
                   double tol, int32_t* slot);
 /* Advance every active lattice by `steps` ANF steps (rounded up to even).
@@ -204,6 +209,11 @@ int skx_batch_remove(skx_batch_handle b, int32_t slot, double* F_out, double* V_
  * milliseconds per copy.  Used as the measured copy peak and to calibrate
  * HBM byte counters against a kernel of known traffic. */
 int skx_diag_stream_copy(int32_t device, size_t bytes, int32_t reps, double* gbs, double* ms_per_copy);
+
+/* Device identity (no reference counterpart): architecture name (e.g.
+ * "gfx950", NUL-terminated, truncated to n bytes) and compute-unit count.
+ * Keys the launch-shape autotuner's cache. */
+int skx_device_info(int32_t device, char* arch, int32_t n, int32_t* cu_count);
 
 const char* skx_last_error(void);
 int skx_version(void);

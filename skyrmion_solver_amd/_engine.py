@@ -37,7 +37,7 @@ EXPORTED = (
     "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_sync",
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
-    "skx_batch_remove", "skx_diag_stream_copy", "skx_last_error", "skx_version",
+    "skx_batch_remove", "skx_diag_stream_copy", "skx_device_info", "skx_last_error", "skx_version",
 )
 
 
@@ -130,6 +130,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_batch_poll": ([H, C.POINTER(SlotStatus), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "skx_batch_remove": ([H, C.c_int32, dp, dp, C.POINTER(SlotStatus)], C.c_int),
         "skx_diag_stream_copy": ([C.c_int32, C.c_size_t, C.c_int32, dp, dp], C.c_int),
+        "skx_device_info": ([C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "skx_last_error": ([], C.c_char_p),
         "skx_version": ([], C.c_int),
     }
@@ -178,6 +179,14 @@ def stream_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20, device: int = 0
     g, ms = C.c_double(), C.c_double()
     _check(lib.skx_diag_stream_copy(device, nbytes, reps, C.byref(g), C.byref(ms)), "skx_diag_stream_copy")
     return g.value, ms.value
+
+
+def device_info(device: int = 0):
+    """(architecture name, compute-unit count) of `device`."""
+    buf = C.create_string_buffer(64)
+    cu = C.c_int32()
+    _check(load_library().skx_device_info(int(device), buf, 64, C.byref(cu)), "skx_device_info")
+    return buf.value.decode(), int(cu.value)
 
 
 class Engine:
@@ -320,9 +329,12 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
-    def add(self, p_i, p_f, F0, V0=None, max_steps: int = 100000, tol: float = -1.0, device: int = 0) -> int:
-        """Add a lattice; raises ``SkxError`` (code -5, SKX_EBUSY) when the batch is full."""
-        d, keep = make_desc(self.theory, self.precision, p_i, p_f, device)
+    def add(self, p_i, p_f, F0, V0=None, max_steps: int = 100000, tol: float = -1.0, device: int = 0,
+            x_chunk: int = 0) -> int:
+        """Add a lattice; raises ``SkxError`` (code -5, SKX_EBUSY) when the batch is full.
+
+        ``x_chunk`` = planes per block along x (0: the whole lattice)."""
+        d, keep = make_desc(self.theory, self.precision, p_i, p_f, device, 0, 0, x_chunk)
         f = np.ascontiguousarray(F0, dtype=np.float64).ravel()
         v = None if V0 is None else np.ascontiguousarray(V0, dtype=np.float64).ravel()
         slot = C.c_int32()

@@ -2,6 +2,7 @@
 // and measures the achievable copy bandwidth of the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "../../include/skx.h"
@@ -54,5 +55,20 @@ extern "C" int skx_diag_stream_copy(int32_t device, size_t bytes, int32_t reps, 
   const double per = ms / reps;
   if (ms_per_copy) *ms_per_copy = per;
   if (gbs) *gbs = 2.0 * (double)(n * sizeof(float4)) / (per * 1e-3) / 1e9;
+  return SKX_OK;
+}
+
+extern "C" int skx_device_info(int32_t device, char* arch, int32_t n, int32_t* cu_count) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SKX_EHIP;
+  if (arch && n > 0) {
+    std::string a(prop.gcnArchName);
+    const size_t colon = a.find(':');  // drop feature suffixes (":sramecc+:xnack-")
+    if (colon != std::string::npos) a.resize(colon);
+    const size_t m = std::min(a.size(), (size_t)n - 1);
+    a.copy(arch, m);
+    arch[m] = '\0';
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
   return SKX_OK;
 }

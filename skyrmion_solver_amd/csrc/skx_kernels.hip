@@ -75,6 +75,7 @@ hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, c
     case PION: return stage_launch_batch<double, PION>(stage, s, slots, blk_inst, parity, grid, st);
     case OMEGA: return stage_launch_batch<double, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
     case COULOMB: return stage_launch_batch<double, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
+    case RHO: return stage_launch_batch<double, RHO>(stage, s, slots, blk_inst, parity, grid, st);
   }
   return hipErrorInvalidValue;
 }
@@ -84,6 +85,7 @@ hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, c
     case PION: return stage_launch_batch<float, PION>(stage, s, slots, blk_inst, parity, grid, st);
     case OMEGA: return stage_launch_batch<float, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
     case COULOMB: return stage_launch_batch<float, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
+    case RHO: return stage_launch_batch<float, RHO>(stage, s, slots, blk_inst, parity, grid, st);
   }
   return hipErrorInvalidValue;
 }

@@ -232,4 +232,21 @@ __global__ void __launch_bounds__(TZ* TY) k_rho_curl(const StageArgs<T> A) {
   rho_curl_body<T, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
 }
 
+// Batched curl pass: the grid is three copies (one per isospin) of the
+// batch's block table; a lattice's blocks of isospin a are numbered
+// a * nblocks + (block - first_block), the order rho_curl_body expects.
+template <typename T, int STAGE, int TZ, int TY>
+__global__ void __launch_bounds__(TZ* TY) k_rho_curl_batch(const BatchSlot<T>* __restrict__ slots,
+                                                          const int* __restrict__ blk_inst, int parity, int grid) {
+  const int iso = (int)blockIdx.x / grid;
+  const int tb = (int)blockIdx.x - iso * grid;
+  const int inst = blk_inst[tb];
+  if (inst < 0) return;
+  const BatchSlot<T>& sl = slots[inst];
+  if (!sl.active) return;
+  if constexpr (STAGE >= 1)
+    rho_curl_body<T, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], iso * sl.nblocks + (tb - sl.first_block),
+                                    3 * sl.nblocks);
+}
+
 }  // namespace skx

@@ -338,6 +338,7 @@ int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, 
   if (int rc = push_status(e)) return rc;
   const int c0 = e->cur;
   int chunk = check_every > 0 ? check_every : 32;
+  // traced runs launch directly: every kernel is bracketed by its own event pair
   const bool use_graph = !(e->flags & (SKX_FLAG_NO_GRAPH | SKX_FLAG_TRACE));
   const int fuse_baryon = e->hist ? 1 : 0;
   if (use_graph && (e->graph_steps != chunk || e->graph_flags != fuse_baryon)) {
@@ -916,8 +917,9 @@ extern "C" {
 int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   if (!d || !out) return fail(SKX_EINVAL, "null argument");
   *out = nullptr;
-  if (d->theory != SKX_THEORY_PION && d->theory != SKX_THEORY_OMEGA && d->theory != SKX_THEORY_COULOMB)
-    return fail(SKX_ENOTSUP, "batches run the pion, omega and Coulomb theories");
+  if (d->theory != SKX_THEORY_PION && d->theory != SKX_THEORY_OMEGA && d->theory != SKX_THEORY_COULOMB &&
+      d->theory != SKX_THEORY_RHO)
+    return fail(SKX_EINVAL, "unknown theory id");
   if (d->precision != SKX_FP64 && d->precision != SKX_FP32) return fail(SKX_EINVAL, "unknown precision");
   if (d->max_slots < 1 || d->max_blocks < 1) return fail(SKX_EINVAL, "max_slots and max_blocks must be positive");
   skx_batch* b = new skx_batch();
@@ -974,8 +976,11 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   if (i == b->max_slots) return fail(SKX_EBUSY, "no free slot in the batch");
   Lattice L;
   if (int rc = parse_desc(lattice, L)) return rc;
-  // one x-chunk per lattice: the batch's other lattices supply the parallelism
-  const int nblk = (int)(((L.Z + b->tile.tz - 1) / b->tile.tz) * (long long)((L.Y + b->tile.ty - 1) / b->tile.ty));
+  // x-chunk: the whole lattice unless the caller asks for shorter marches
+  // (the batch's other lattices supply the parallelism)
+  const int xc = (lattice->x_chunk > 0 && lattice->x_chunk < L.X) ? lattice->x_chunk : L.X;
+  const int nblk = (int)(((L.Z + b->tile.tz - 1) / b->tile.tz) * (long long)((L.Y + b->tile.ty - 1) / b->tile.ty) *
+                         ((L.X + xc - 1) / xc));
   int first = -1;
   for (int s = 0, run = 0; s < b->max_blocks; ++s) {  // first fit in the block table
     run = (b->h_blk[s] < 0) ? run + 1 : 0;
@@ -988,7 +993,7 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   skx_desc d = *lattice;
   d.tile_z = b->tile.tz;
   skx_engine* e = nullptr;
-  if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, L.X, &e)) return rc;
+  if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, xc, &e)) return rc;
   const size_t n = (size_t)L.N * L.nf;
   if (int rc = copy_in(e, e->F[0], F0, n)) {
     release(e);

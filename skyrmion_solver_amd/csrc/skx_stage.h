@@ -353,37 +353,44 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
 #pragma unroll
           for (int a = 0; a < NF; ++a) bstore<T>(-dt * g[a], plane_rsrc(A.lout + a * N + (size_t)c * YZ, plane_bytes), own_b);
         } else {
-          // RK4 combine (core/integrator.py:170-175), recomputing k1..k4 from V and l1..l3
-          using R = double;
+          // RK4 combine (core/integrator.py:170-175), recomputing k1..k4 from V and l1..l3,
+          // in the storage precision: the result is rounded to it when stored
+          using R = T;
           R Fn[NF], Vn[NF];
           const R rdt = R(A.P.dt);
 #pragma unroll
           for (int a = 0; a < NF; ++a) {
-            const R Fv = R(cF[a]);
-            const R Vk = (keep != T(0)) ? R(cV[a]) : R(0);
-            const R l1 = R(cl1[a]), l2 = R(cl2[a]), l3 = R(cl3[a]);
-            const R l4 = R(-dt * g[a]);
+            const R Fv = cF[a];
+            const R Vk = (keep != T(0)) ? cV[a] : R(0);
+            const R l1 = cl1[a], l2 = cl2[a], l3 = cl3[a];
+            const R l4 = -dt * g[a];
             const R k1 = rdt * Vk;
-            const R k2 = rdt * (Vk + 0.5 * l1);
-            const R k3 = rdt * (Vk + 0.5 * l2);
+            const R k2 = rdt * (Vk + R(0.5) * l1);
+            const R k3 = rdt * (Vk + R(0.5) * l2);
             const R k4 = rdt * (Vk + l3);
-            Vn[a] = Vk + (1.0 / 6.0) * (l1 + 2.0 * l2 + 2.0 * l3 + l4);
-            Fn[a] = Fv + (1.0 / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+            Vn[a] = Vk + R(1.0 / 6.0) * (l1 + R(2.0) * l2 + R(2.0) * l3 + l4);
+            Fn[a] = Fv + R(1.0 / 6.0) * (k1 + R(2.0) * k2 + R(2.0) * k3 + k4);
             const double gv = double(g[a]);
             acc0 = fmax(acc0, gv);
             acc1 = fmax(acc1, fabs(gv));
             bad |= !isfinite(gv);
           }
           if (A.unit_length) {
-            R s = 0.0;
+            R s = R(0);
 #pragma unroll
             for (int a = 0; a < NSK; ++a) s += Fn[a] * Fn[a];
             s = sqrt(s);
-            if (s != 0.0) {
+            if (s != R(0)) {
+              if constexpr (sizeof(T) == 8) {  // FP64: divide like compute_norm_skyrme_field
 #pragma unroll
-              for (int a = 0; a < NSK; ++a) Fn[a] /= s;
+                for (int a = 0; a < NSK; ++a) Fn[a] /= s;
+              } else {
+                const R inv = R(1) / s;
+#pragma unroll
+                for (int a = 0; a < NSK; ++a) Fn[a] *= inv;
+              }
             }
-            R lm = 0.0;
+            R lm = R(0);
 #pragma unroll
             for (int a = 0; a < NSK; ++a) lm += Vn[a] * Fn[a];
 #pragma unroll
@@ -462,8 +469,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   }
 }
 
+// Waves per SIMD the register allocation must leave room for: 3 for the
+// FP32 pion stage 2, which fits in 168 VGPRs without spilling (three 46 KB
+// blocks per CU); no constraint elsewhere (the others would spill).
+template <typename T, int TH, int STAGE>
+constexpr int stage_min_waves() {
+  return (sizeof(T) == 4 && STAGE == 2 && TH == PION) ? 3 : 1;
+}
+
 template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY) k_stage(const StageArgs<T> A) {
+__global__ void __launch_bounds__(TZ* TY) __attribute__((amdgpu_waves_per_eu(stage_min_waves<T, TH, STAGE>())))
+k_stage(const StageArgs<T> A) {
   stage_body<T, TH, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
 }
 
@@ -471,7 +487,8 @@ __global__ void __launch_bounds__(TZ* TY) k_stage(const StageArgs<T> A) {
 // grid.  blk_inst maps every block of the grid to a slot (-1 = idle block);
 // the slot carries the lattice's arguments for both ping-pong parities.
 template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY) k_stage_batch(const BatchSlot<T>* __restrict__ slots,
+__global__ void __launch_bounds__(TZ* TY) __attribute__((amdgpu_waves_per_eu(stage_min_waves<T, TH, STAGE>())))
+k_stage_batch(const BatchSlot<T>* __restrict__ slots,
                                                        const int* __restrict__ blk_inst, int parity) {
   const int inst = blk_inst[blockIdx.x];
   if (inst < 0) return;
@@ -533,6 +550,12 @@ struct StageBatchLaunchFn {
   hipStream_t st;
   template <int STAGE, int TZ, int TY>
   hipError_t operator()() {
+    if constexpr (TH == RHO) {  // curl pass of every lattice's three isospins first
+      hipLaunchKernelGGL((k_rho_curl_batch<T, STAGE, TZ, TY>), dim3(3 * grid), dim3(TZ * TY), 0, st, slots,
+                         blk_inst, parity, grid);
+      const hipError_t r = hipGetLastError();
+      if (r != hipSuccess) return r;
+    }
     hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY>), dim3(grid), dim3(TZ * TY), 0, st, slots, blk_inst,
                        parity);
     return hipGetLastError();
@@ -570,10 +593,6 @@ hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks) {
   template hipError_t stage_launch<T, TH>(int, const StageArgs<T>&, const LaunchShape&, hipStream_t);          \
   template hipError_t stage_launch_batch<T, TH>(int, const LaunchShape&, const BatchSlot<T>*, const int*, int, \
                                                 int, hipStream_t);                                             \
-  template hipError_t stage_occupancy<T, TH>(int, const LaunchShape&, int*);
-// rho runs solo engines only (no batched launches)
-#define SKX_INSTANTIATE_STAGES_SOLO(T, TH)                                                            \
-  template hipError_t stage_launch<T, TH>(int, const StageArgs<T>&, const LaunchShape&, hipStream_t); \
   template hipError_t stage_occupancy<T, TH>(int, const LaunchShape&, int*);
 
 }  // namespace skx

@@ -2,5 +2,5 @@
 #include "skx_stage.h"
 
 namespace skx {
-SKX_INSTANTIATE_STAGES_SOLO(float, RHO)
+SKX_INSTANTIATE_STAGES(float, RHO)
 }  // namespace skx

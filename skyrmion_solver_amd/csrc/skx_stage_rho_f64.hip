@@ -2,5 +2,5 @@
 #include "skx_stage.h"
 
 namespace skx {
-SKX_INSTANTIATE_STAGES_SOLO(double, RHO)
+SKX_INSTANTIATE_STAGES(double, RHO)
 }  // namespace skx

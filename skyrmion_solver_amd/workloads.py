@@ -1,10 +1,12 @@
 """Synthetic benchmark workloads (seeded host initial data + packed parameters).
 
-Each builder returns ``(theory_module, resolved_params, p_i, p_f, F0)``.  The
-time step is the reference's example setting ``courant = 0.2``
-(``examples/nuclear_skyrme_gl.py:13``): the larger dt = 0.5 dx these
-workloads were specified with is beyond the RK4 stability limit of the
-reference's own scheme (the run goes non-finite within tens of steps).
+Each single-lattice builder returns ``(theory_module, resolved_params, p_i,
+p_f, F0)``.  The time step is the reference's example setting
+``courant = 0.2`` (``examples/nuclear_skyrme_gl.py:13``): the larger
+dt = 0.5 dx these workloads were specified with is beyond the RK4 stability
+limit of the reference's own scheme (the run goes non-finite within tens of
+steps).  The rho theory takes ``courant = 0.1``: its Laplacian enters the
+gradient with weight 8 (the pion's with 2), which halves the stable step.
 """
 
 from __future__ import annotations
@@ -15,11 +17,15 @@ from skyrmion_solver_amd import initial_data
 from skyrmion_solver_amd.theories import load_theory
 
 COURANT = 0.2
+COURANT_BY_THEORY = {"pion": 0.2, "omega": 0.2, "coulomb": 0.2, "rho": 0.1}
+THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model", "coulomb": "Coulomb-Skyrme model",
+                "rho": "Rho-Skyrme model"}
+SERVE_MIX = (("pion", 0.60), ("omega", 0.25), ("rho", 0.15))
 
 
-def _lattice(L, dx):
+def _lattice(L, dx, courant=COURANT):
     return dict(xlen=L, ylen=L, zlen=L, lsx=dx, lsy=dx, lsz=dx, xsize=dx * (L - 1), ysize=dx * (L - 1),
-                zsize=dx * (L - 1), courant=COURANT)
+                zsize=dx * (L - 1), courant=courant)
 
 
 def c1_hedgehog(seed: int = 0, L: int = 32):
@@ -92,3 +98,50 @@ def set_omega_from_chern_simons(engine, rp):
     F[4 * N:5 * N] = -float(rp.c_omega) * (cs_vol / float(rp.grid_volume))
     engine.upload(_engine.FIELD, F)
     return F
+
+
+def serving_request(rid: int, theory: str, L: int, B: int, rng, dx: float = 0.15, tol: float = 1.0e-4,
+                    max_steps: int = 20000, arrival: float = 0.0):
+    """One minimisation request: a B = 1..8 rational map of random space / isospin orientation on an L^3
+    lattice with seeded parameters (pion m ~ U[0.5, 1.5]; omega c_omega ~ U[0.3, 0.7]; rho m_pi ~
+    U[0.5, 1.5], m_rho ~ U[1, 2], c_alpha ~ U[0, 0.3]); omega and rho start at 0."""
+    from skyrmion_solver_amd.serving import Request
+
+    th = load_theory(THEORY_NAMES[theory])
+    lat = _lattice(L, dx, COURANT_BY_THEORY[theory])
+    if theory == "pion":
+        extra = dict(mpi=float(rng.uniform(0.5, 1.5)), kappa=1.0)
+    elif theory == "omega":
+        extra = dict(c_omega=float(rng.uniform(0.3, 0.7)))
+    elif theory == "rho":
+        extra = dict(mpi=float(rng.uniform(0.5, 1.5)), mrho=float(rng.uniform(1.0, 2.0)),
+                     c_alpha=float(rng.uniform(0.0, 0.3)))
+    else:
+        extra = dict(kappa=0.1)
+    rp = th.params.default_params(**lat, baryon_number=float(B), **extra).resolved()
+    p_i, p_f = th.params.pack_device_params(rp)
+    nf = int(p_i[5])
+    F = initial_data.build_field(rp, nf, dict(mode="rational_map", baryon_number=B, profile=("exp", 1.0 + 0.15 * B),
+                                             isorotation="haar", space_rotation="haar",
+                                             seed=int(rng.integers(1 << 30)), vacuum_band=True))
+    return Request(rid=rid, theory=theory, p_i=p_i, p_f=p_f, F0=F, tol=tol, max_steps=max_steps, arrival=arrival)
+
+
+def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 128, 160, 192), mix=SERVE_MIX,
+                tol: float = 1.0e-4, max_steps: int = 20000, dx: float = 0.15):
+    """Poisson stream of `n` requests at `rate` per second: theory drawn from `mix`, L from `sizes`, B = 1..8."""
+    from skyrmion_solver_amd.serving import poisson_arrivals
+
+    rng = np.random.default_rng(seed)
+    arrivals = poisson_arrivals(n, rate, seed)
+    names = [m[0] for m in mix]
+    probs = np.array([m[1] for m in mix], dtype=float)
+    probs /= probs.sum()
+    out = []
+    for k in range(n):
+        theory = str(rng.choice(names, p=probs))
+        L = int(rng.choice(sizes))
+        B = int(rng.integers(1, 9))
+        out.append(serving_request(k, theory, L, B, rng, dx=dx, tol=tol, max_steps=max_steps,
+                                   arrival=float(arrivals[k])))
+    return out

@@ -17,13 +17,14 @@ from skyrmion_solver_amd import _engine
 
 pytestmark = pytest.mark.gpu
 
-TID = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB}
+TID = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
+       "rho": _engine.THEORY_RHO}
 
 
 def lattices(theory, shapes, seed0=0):
     out = []
     for k, sh in enumerate(shapes):
-        kw = {"mpi": 0.5 + 0.2 * k} if theory == "pion" else {"mass": 0.4 + 0.1 * k}
+        kw = {"mpi": 0.5 + 0.2 * k} if theory in ("pion", "rho") else {"mass": 0.4 + 0.1 * k}
         mod, rp, p_i, p_f = make_params(theory, *sh, **kw)
         F = smooth_field(rp, int(p_i[5]), seed=seed0 + k, normalise=True)
         out.append((p_i, p_f, F))
@@ -38,7 +39,8 @@ def solo(theory, prec, p_i, p_f, F, steps, tol):
 
 
 @pytest.mark.parametrize("theory,prec", [("pion", _engine.FP64), ("pion", _engine.FP32),
-                                         ("coulomb", _engine.FP32), ("omega", _engine.FP64)])
+                                         ("coulomb", _engine.FP32), ("omega", _engine.FP64),
+                                         ("rho", _engine.FP64), ("rho", _engine.FP32)])
 def test_batch_equals_solo_bitwise(theory, prec):
     print(f"[case] batch theory={theory} prec={prec}", flush=True)
     lat = lattices(theory, [(20, 18, 16), (16, 16, 16), (24, 21, 34)])
