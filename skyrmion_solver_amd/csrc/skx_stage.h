@@ -92,7 +92,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   constexpr int TW = TZ + 4;  // tile row length (z with halo)
   constexpr int TR = TY + 4;  // tile rows (y with halo)
   constexpr int TE = TW * TR;
-  constexpr int LPT = (TE + NT - 1) / NT;
+  constexpr int NH = TE - NT;                  // halo cells of the tile
+  constexpr int LPT = 1 + (NH + NT - 1) / NT;  // elements staged per thread: its own site + a halo share
   constexpr int NA = (STAGE <= 1) ? 1 : (STAGE == 2 ? 2 : 3);
   constexpr bool D2 = STAGE >= 1;
   constexpr bool ANY_MIXED = D2 && (Tr::MIXED != 0u);
@@ -101,7 +102,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   static_assert(NT % 64 == 0, "block must be whole wavefronts");
   static_assert(TY >= 4, "halo rows of the z-difference ring are filled by the first four rows of threads");
 
-  __shared__ T fring[FSLOTS][NST][TE];
+  __shared__ T fring[FSLOTS][NST][TE];  // (rawq: declared below, stage 4 only)
   __shared__ T zring[ANY_MIXED ? ZSLOTS : 1][ANY_MIXED ? NST : 1][ANY_MIXED ? TR * TZ : 1];
   __shared__ double red[2 * (NT / 64)];
 
@@ -142,15 +143,37 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   const T mxz_a = gz2 ? cx.d1a : T(0), mxz_b = gz2 ? cx.d1b : T(0);
   const T myz_a = (gy2 && gz2) ? cy.d1a : T(0), myz_b = (gy2 && gz2) ? cy.d1b : T(0);
 
-  // tile elements this thread stages: byte offsets within a plane, kOOB for
-  // cells outside the lattice (they load as 0)
+  // Tile elements this thread stages: element 0 is its own site (tile row
+  // ty + 2, column tz + 2), so the raw values of its own column arrive in its
+  // own registers; elements 1.. are halo cells h = tid + (j - 1) NT, numbered
+  // rows 0, 1, TR-2, TR-1 in full, then columns 0, 1, TW-2, TW-1 of the rows
+  // between.  eidx = LDS index (-1: no element), eoff = byte offset in the
+  // plane (kOOB outside the lattice: loads 0).
+  int eidx[LPT];
   uint32_t eoff[LPT];
 #pragma unroll
   for (int j = 0; j < LPT; ++j) {
-    const int e = tid + j * NT;
-    const int r = e / TW, cc = e - r * TW;
+    int r = ty + 2, cc = tz + 2;
+    bool in_tile = true;
+    if (j > 0) {
+      const int h = tid + (j - 1) * NT;
+      in_tile = h < NH;
+      if (h < 2 * TW) {
+        r = h / TW;
+        cc = h - r * TW;
+      } else if (h < 4 * TW) {
+        const int m = h - 2 * TW;
+        r = TR - 2 + m / TW;
+        cc = m % TW;
+      } else {
+        const int m = h - 4 * TW, q = m & 3;
+        r = 2 + m / 4;
+        cc = q < 2 ? q : TW - 4 + q;
+      }
+    }
     const int gy = y0 - 2 + r, gz = z0 - 2 + cc;
-    const bool valid = (e < TE) && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
+    const bool valid = in_tile && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
+    eidx[j] = in_tile ? r * TW + cc : -1;
     eoff[j] = valid ? (uint32_t)(gy * Z + gz) * (uint32_t)sizeof(T) : kOOB;
   }
   const uint32_t own_off = own ? (uint32_t)(y * Z + z) : 0u;
@@ -187,6 +210,18 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   for (int a = 0; a < NST; ++a)
 #pragma unroll
     for (int k = 0; k < 5; ++k) qf[a][k] = qy[a][k] = qz[a][k] = T(0);
+  // stage 4: raw F, V, l2 of the own column are parked in a private LDS ring
+  // from their plane's arrival until it is the centre, so the RK4 combine
+  // does not read them again from memory.  Slot p & 1 holds plane p - 2 when
+  // plane p arrives: it is read, then overwritten with plane p.  Each thread
+  // touches only its own entries, so the ring needs no barrier.  Used where it
+  // fits in the 160 KB of LDS beside the plane rings (all FP32 shapes; FP64
+  // pion 32x8 and 16x16, where one block per CU is resident anyway).
+  constexpr size_t LDS_RINGS = sizeof(T) * (size_t(FSLOTS) * NST * TE + (ANY_MIXED ? size_t(ZSLOTS) * NST * TR * TZ : 0)) +
+                               2 * (NT / 64) * sizeof(double);
+  constexpr size_t LDS_RAWQ = sizeof(T) * 2 * 3 * size_t(NST) * NT;
+  constexpr bool RAWQ = (STAGE == 4) && (sizeof(T) == 4 || LDS_RINGS + LDS_RAWQ <= 160 * 1024);
+  __shared__ T rawq[RAWQ ? 2 : 1][RAWQ ? 3 * NST : 1][RAWQ ? NT : 1];
 
   double acc0 = (STAGE == 4) ? -1.0e300 : 0.0;
   double acc1 = 0.0;
@@ -204,28 +239,43 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     for (int a = 0; a < NST; ++a)
 #pragma unroll
       for (int j = 0; j < LPT; ++j) {
-        const int e = tid + j * NT;
-        if (j < LPT - 1 || e < TE)
-          fring[fs][a][e] = stage_value<STAGE>(pre[0][a][j], NA > 1 ? pre[NA > 1 ? 1 : 0][a][j] : T(0),
-                                               NA > 2 ? pre[NA > 2 ? 2 : 0][a][j] : T(0), dt, keep);
+        if (j == 0 || eidx[j] >= 0)
+          fring[fs][a][eidx[j]] = stage_value<STAGE>(pre[0][a][j], NA > 1 ? pre[NA > 1 ? 1 : 0][a][j] : T(0),
+                                                     NA > 2 ? pre[NA > 2 ? 2 : 0][a][j] : T(0), dt, keep);
       }
+    T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
+    if constexpr (RAWQ) {
+      const int rs = p & 1;
+#pragma unroll
+      for (int a = 0; a < NST; ++a) {
+        cF[a] = rawq[rs][a][tid];
+        cV[a] = rawq[rs][NST + a][tid];
+        cl2[a] = rawq[rs][2 * NST + a][tid];
+        rawq[rs][a][tid] = pre[0][a][0];
+        rawq[rs][NST + a][tid] = pre[1][a][0];
+        rawq[rs][2 * NST + a][tid] = pre[NA > 2 ? 2 : 0][a][0];
+      }
+    }
     __syncthreads();
     if (p + 1 <= xe + 1) issue(p + 1);
 
     const int c = p - 2;
     const bool center = (c >= xs) && own;
-    // stage 4 reads its own site's F, V, l1, l2, l3 for the RK4 combine: issue early
-    T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
+    // stage 4: its own site's F, V, l1, l2, l3 for the RK4 combine -- F, V, l2
+    // of the stencil components came from the raw ring above (FP32), the rest
+    // are loaded here (issued early)
     if constexpr (STAGE == 4) {
       if (c >= xs) {
         const size_t pb = (size_t)c * YZ;
 #pragma unroll
         for (int a = 0; a < NF; ++a) {
           const size_t o = a * N + pb;
-          cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), own_b);
-          cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), own_b);
+          if (!RAWQ || a >= NST) {
+            cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), own_b);
+            cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), own_b);
+            cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), own_b);
+          }
           cl1[a] = bload<T>(plane_rsrc(A.lb + o, plane_bytes), own_b);
-          cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), own_b);
           cl3[a] = bload<T>(plane_rsrc(A.lc + o, plane_bytes), own_b);
         }
       }
@@ -376,25 +426,36 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
             bad |= !isfinite(gv);
           }
           if (A.unit_length) {
-            R s = R(0);
+            // S^3 renormalisation of F and tangent projection of V in FP64 in both forms
+            // (compute_norm_skyrme_field / project_orthogonal_skyrme_field,
+            // theories/nuclear_skyrme/kernels.py:196-272)
+            double f[NSK], v[NSK], s = 0.0;
 #pragma unroll
-            for (int a = 0; a < NSK; ++a) s += Fn[a] * Fn[a];
+            for (int a = 0; a < NSK; ++a) {
+              f[a] = double(Fn[a]);
+              v[a] = double(Vn[a]);
+              s += f[a] * f[a];
+            }
             s = sqrt(s);
-            if (s != R(0)) {
-              if constexpr (sizeof(T) == 8) {  // FP64: divide like compute_norm_skyrme_field
+            if (s != 0.0) {
+              if constexpr (sizeof(T) == 8) {  // FP64 form: one division per component, as the reference
 #pragma unroll
-                for (int a = 0; a < NSK; ++a) Fn[a] /= s;
-              } else {
-                const R inv = R(1) / s;
+                for (int a = 0; a < NSK; ++a) f[a] /= s;
+              } else {  // FP32 form: the result is rounded to FP32 on store
+                const double inv = 1.0 / s;
 #pragma unroll
-                for (int a = 0; a < NSK; ++a) Fn[a] *= inv;
+                for (int a = 0; a < NSK; ++a) f[a] *= inv;
               }
             }
-            R lm = R(0);
+            double lm = 0.0;
 #pragma unroll
-            for (int a = 0; a < NSK; ++a) lm += Vn[a] * Fn[a];
+            for (int a = 0; a < NSK; ++a) lm += v[a] * f[a];
 #pragma unroll
-            for (int a = 0; a < NSK; ++a) Vn[a] -= lm * Fn[a];
+            for (int a = 0; a < NSK; ++a) {
+              v[a] -= lm * f[a];
+              Fn[a] = R(f[a]);
+              Vn[a] = R(v[a]);
+            }
           }
 #pragma unroll
           for (int a = 0; a < NF; ++a) {

@@ -291,6 +291,38 @@ def set_vacuum_band(F, width=2):
     return F
 
 
+def first_derivative(f: np.ndarray, axis: int, h: float, halo: int = 2) -> np.ndarray:
+    """4th-order central first derivative of the 3D array `f` along `axis`, zero within `halo` sites of
+    either end (the stencil and band of ``compute_derivative_first``, core/derivatives.py:17-67)."""
+    n = f.shape[axis]
+    out = np.zeros_like(f)
+    lo, hi = halo, n - halo
+    if hi <= lo:
+        return out
+
+    def sl(a, b):
+        idx = [slice(None)] * 3
+        idx[axis] = slice(a, b)
+        return tuple(idx)
+
+    out[sl(lo, hi)] = (f[sl(lo - 2, hi - 2)] - 8.0 * f[sl(lo - 1, hi - 1)] + 8.0 * f[sl(lo + 1, hi + 1)]
+                       - f[sl(lo + 2, hi + 2)]) / (12.0 * h)
+    return out
+
+
+def chern_simons_density(F: np.ndarray, rp) -> np.ndarray:
+    """CS = -det[phi, d_x phi, d_y phi, d_z phi] / (2 pi^2) of the 4 pion components of `F` (nf, X, Y, Z),
+    the baryon density without the cell volume (``_chern_simons_current_point``,
+    theories/omega_skyrme/kernels.py:443-500)."""
+    halo = int(getattr(rp, "halo", 2))
+    phi = F[:4]
+    d = [np.stack([first_derivative(phi[a], ax, h, halo) for a in range(4)])
+         for ax, h in ((0, rp.lsx), (1, rp.lsy), (2, rp.lsz))]
+    M = np.stack([phi, d[0], d[1], d[2]])  # (column, component, X, Y, Z)
+    det = np.linalg.det(np.moveaxis(M, (0, 1), (-1, -2)))  # rows = components, columns = (phi, dx, dy, dz)
+    return -det / (2.0 * np.pi * np.pi)
+
+
 def build_field(rp, nf: int, config: dict | None = None) -> np.ndarray:
     """Initial field ``(nf, X, Y, Z)`` for an ``initialize(config)`` request.
 

@@ -100,35 +100,48 @@ def set_omega_from_chern_simons(engine, rp):
     return F
 
 
-def serving_request(rid: int, theory: str, L: int, B: int, rng, dx: float = 0.15, tol: float = 1.0e-4,
-                    max_steps: int = 20000, arrival: float = 0.0):
+def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0e-4, max_steps: int = 20000,
+                    arrival: float = 0.0):
     """One minimisation request: a B = 1..8 rational map of random space / isospin orientation on an L^3
-    lattice with seeded parameters (pion m ~ U[0.5, 1.5]; omega c_omega ~ U[0.3, 0.7]; rho m_pi ~
-    U[0.5, 1.5], m_rho ~ U[1, 2], c_alpha ~ U[0, 0.3]); omega and rho start at 0."""
+    lattice, in each theory's working units (the reference's examples):
+
+    * pion: dx = 0.15, courant 0.2, m ~ U[0.5, 1.5], kappa = 1;
+    * omega: dx = 12/63, courant 0.2, f_pi = 139.8, m_pi = 43.91, m_omega = 249.5, beta_omega ~ U[7.5, 8.5]
+      (c_omega ~ 14, examples/omega_skyrme_gl.py:11-16), omega started at -c_omega CS;
+    * rho: dx = 2/63, courant 0.05, e = 3.65, f_pi = 129, m_pi = 138, m_rho = 775, alpha ~ U[0.055, 0.068]
+      (examples/rho_skyrme_gl.py:9-15), rho started at 0.
+    """
     from skyrmion_solver_amd.serving import Request
 
     th = load_theory(THEORY_NAMES[theory])
-    lat = _lattice(L, dx, COURANT_BY_THEORY[theory])
     if theory == "pion":
+        lat = _lattice(L, 0.15, 0.2)
         extra = dict(mpi=float(rng.uniform(0.5, 1.5)), kappa=1.0)
     elif theory == "omega":
-        extra = dict(c_omega=float(rng.uniform(0.3, 0.7)))
+        lat = _lattice(L, 12.0 / 63.0, 0.2)
+        extra = dict(f_pi=139.8, m_pi=43.91, m_omega=249.5, beta_omega=float(rng.uniform(7.5, 8.5)))
     elif theory == "rho":
-        extra = dict(mpi=float(rng.uniform(0.5, 1.5)), mrho=float(rng.uniform(1.0, 2.0)),
-                     c_alpha=float(rng.uniform(0.0, 0.3)))
+        lat = _lattice(L, 2.0 / 63.0, 0.05)
+        extra = dict(e_skyrme=3.65, f_pi=129.0, mass_pion=138.0, mass_rho=775.0, alpha=float(rng.uniform(0.055, 0.068)))
     else:
+        lat = _lattice(L, 0.15, 0.2)
         extra = dict(kappa=0.1)
     rp = th.params.default_params(**lat, baryon_number=float(B), **extra).resolved()
     p_i, p_f = th.params.pack_device_params(rp)
     nf = int(p_i[5])
-    F = initial_data.build_field(rp, nf, dict(mode="rational_map", baryon_number=B, profile=("exp", 1.0 + 0.15 * B),
+    # profile length: the pion / omega units hold a soliton of radius ~1; the rho box is 2 units wide, so its
+    # profile scales with the box (as the reference's hedgehog does, r0 = rmax / (8 B))
+    a = (1.0 + 0.15 * B) if theory != "rho" else float(rp.xsize) * (1.0 + 0.15 * B) / 14.0
+    F = initial_data.build_field(rp, nf, dict(mode="rational_map", baryon_number=B, profile=("exp", a),
                                              isorotation="haar", space_rotation="haar",
                                              seed=int(rng.integers(1 << 30)), vacuum_band=True))
+    if theory == "omega":
+        F[4] = -float(rp.c_omega) * initial_data.chern_simons_density(F, rp)
     return Request(rid=rid, theory=theory, p_i=p_i, p_f=p_f, F0=F, tol=tol, max_steps=max_steps, arrival=arrival)
 
 
 def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 128, 160, 192), mix=SERVE_MIX,
-                tol: float = 1.0e-4, max_steps: int = 20000, dx: float = 0.15):
+                tol: float = 1.0e-4, max_steps: int = 20000):
     """Poisson stream of `n` requests at `rate` per second: theory drawn from `mix`, L from `sizes`, B = 1..8."""
     from skyrmion_solver_amd.serving import poisson_arrivals
 
@@ -142,6 +155,5 @@ def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 
         theory = str(rng.choice(names, p=probs))
         L = int(rng.choice(sizes))
         B = int(rng.integers(1, 9))
-        out.append(serving_request(k, theory, L, B, rng, dx=dx, tol=tol, max_steps=max_steps,
-                                   arrival=float(arrivals[k])))
+        out.append(serving_request(k, theory, L, B, rng, tol=tol, max_steps=max_steps, arrival=float(arrivals[k])))
     return out

@@ -78,3 +78,20 @@ def test_invalid_descriptor_is_rejected_without_a_device():
     assert e.value.code == -1
     n = _engine.footprint_bytes(_engine.THEORY_PION, _engine.FP32, p_i, p_f)
     assert n >= 7 * 4 * 512 * 4
+
+
+def test_host_chern_simons_density_matches_oracle():
+    """initial_data.chern_simons_density (omega starts at -c CS) against the oracle's CS of the same field."""
+    import numpy as np
+
+    from helpers import make_params, smooth_field
+    from oracle import stencils as OS
+    from oracle import theories as OT
+    from skyrmion_solver_amd import initial_data
+
+    mod, rp, p_i, p_f = make_params("omega", 14, 13, 12)
+    T = smooth_field(rp, 5, seed=3, normalise=True)
+    d1 = OS.first_derivatives(T, (rp.lsx, rp.lsy, rp.lsz))
+    ref = OT.chern_simons(T, d1)
+    got = initial_data.chern_simons_density(T, rp)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
