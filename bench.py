@@ -136,7 +136,7 @@ def batch_sample(n: int, steps: int):
     one after another as solo engines with the engine's default launch shape."""
     from skyrmion_solver_amd import autotune, workloads
 
-    inst = workloads.c4_instances(n=n, seed=0)
+    inst = workloads.c4_instances(n=n, seed=0, device_init=True)
     per_class = {}
     for d in inst:
         per_class.setdefault(d["L"], d)
@@ -147,8 +147,8 @@ def batch_sample(n: int, steps: int):
     blocks = autotune.batch_blocks(dims, tz, tgt)
     sites = sum(d["L"] ** 3 for d in inst)
     with _engine.Batch(_engine.THEORY_PION, _engine.FP32, max_slots=len(inst), max_blocks=blocks, tile_z=tz) as b:
-        for d in inst:
-            b.add(d["p_i"], d["p_f"], d["F"], max_steps=1 << 30, x_chunk=autotune.chunk_rule(d["L"], tgt))
+        for d in inst:  # fields built on the device from each lattice's initial-data descriptor
+            b.add(d["p_i"], d["p_f"], init=d["init"], max_steps=1 << 30, x_chunk=autotune.chunk_rule(d["L"], tgt))
         b.run(4)
         t0 = time.perf_counter()
         b.run(steps)
@@ -158,7 +158,7 @@ def batch_sample(n: int, steps: int):
     for L, d in per_class.items():
         count = sum(1 for e in inst if e["L"] == L)
         eng = _engine.Engine(_engine.THEORY_PION, _engine.FP32, d["p_i"], d["p_f"])
-        eng.upload(_engine.FIELD, d["F"])
+        eng.init_field(d["init"])
         eng.anf_run(4, -1.0, 4)
         st = eng.anf_run(steps, -1.0, steps)
         eng.close()
@@ -176,12 +176,11 @@ def serving_sample(n: int, rate: float, sizes, max_steps: int):
 
     out = {}
     for name, prec in (("fp32", _engine.FP32), ("fp64", _engine.FP64)):
-        reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps)
+        reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps, device_init=True)
         srv = serving.Server(precision=prec, hbm_budget_bytes=64 << 30, chunk_steps=32, max_slots=64,
                              max_blocks=32768)
         rep = srv.serve(reqs)
         srv.close()
-        rep["by_theory"] = {t: sum(1 for r in reqs if r.theory == t) for t in ("pion", "omega", "rho")}
         out[name] = rep
     out.update(requests=n, rate_per_s=rate, sizes=list(sizes), max_steps=max_steps)
     return out

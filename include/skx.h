@@ -147,6 +147,47 @@ int skx_density(skx_handle h, int32_t kind, double* host, size_t n);
  * (either may be NULL). */
 int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n);
 
+/* ---------------------------------------------------------------- initial data
+ * Device-side initial configurations.  Replaces the host-built field of
+ * Simulation.initialize -> theory initialize (core/simulation.py:124-173) with
+ * the kernels of theories/nuclear_skyrme/initial_config.py:701-1126
+ * (hedgehog_field, create_initial_configuration_kernel's rational-map branch,
+ * product_ansatz_kernel) and, for the omega theory, create_omega_field_kernel
+ * (theories/omega_skyrme/initial_config.py:1495-1532). */
+enum skx_lump_kind { SKX_LUMP_HEDGEHOG = 0, SKX_LUMP_RATIONAL_MAP = 1 };
+enum skx_profile_kind {
+  SKX_PROFILE_EXP = 0,   /* f = pi exp(-r / a),      a = profile_param */
+  SKX_PROFILE_GAUSS = 1, /* f = pi exp(-(r / a)^2),  a = profile_param */
+  SKX_PROFILE_FIT = 2    /* profilefun(r, B, m0, 0) (initial_config.py:343-442), m0 = profile_param */
+};
+#define SKX_MAX_LUMPS 8
+
+typedef struct skx_lump {
+  int32_t kind;          /* skx_lump_kind */
+  int32_t baryon_number; /* rational map R(z) for B = 1..8 (initial_config.py:446-549), z^B above */
+  int32_t profile;       /* skx_profile_kind */
+  int32_t pad_;
+  double profile_param;
+  double center[3];      /* lump centre relative to the lattice centre */
+  double space_rot[9];   /* row-major S: position r -> S (r - center) (rational map only) */
+  double iso_rot[9];     /* row-major R: unit isovector n -> R n */
+} skx_lump;
+
+typedef struct skx_init_desc {
+  int32_t n_lumps;           /* 0: vacuum; 1: one lump; > 1: quaternion product of the lumps, in order,
+                                renormalised after each product */
+  int32_t vacuum_band;       /* write the vacuum into the halo-wide boundary band */
+  double noise;              /* N(0, noise) added to each sigma-model component before renormalising
+                                (counter-based generator keyed by noise_seed) */
+  uint64_t noise_seed;
+  double omega_cs;           /* omega theory: omega = -omega_cs * CS of the new pion field (0: omega = 0) */
+  skx_lump lumps[SKX_MAX_LUMPS];
+} skx_init_desc;
+
+/* Set the field to the described configuration (components beyond the 4
+ * sigma-model ones 0, or omega = -c CS) and the velocity to 0. */
+int skx_init_field(skx_handle h, const skx_init_desc* desc);
+
 /* Attach a device history of up to `capacity` skx_step_record entries for the
  * next runs (0 detaches); skx_history_read copies min(n, steps) records. */
 int skx_history_attach(skx_handle h, int32_t capacity);
@@ -186,7 +227,7 @@ typedef struct skx_slot_status {
 
 int skx_batch_create(const skx_batch_desc* desc, skx_batch_handle* out);
 int skx_batch_destroy(skx_batch_handle b);
-/* Add a lattice (F0 / V0 host FP64, V0 may be NULL); it runs until its
+/* Add a lattice (F0 / V0 host FP64; F0 NULL = the vacuum, V0 NULL = 0); it runs until its
  * stage-4 max|g| < tol or max_steps.  lattice->x_chunk = planes per block
  * along x (0: the whole lattice, one block per (y, z) tile); lattice->tile_z
  * is ignored (the batch's tile applies).  SKX_EBUSY when no slot / blocks are
@@ -203,6 +244,9 @@ int skx_batch_poll(skx_batch_handle b, skx_slot_status* out, int32_t n, int32_t*
 /* Finish a lattice (final energy, pending arrest) and copy its field and
  * velocity out (either may be NULL); frees its slot. */
 int skx_batch_remove(skx_batch_handle b, int32_t slot, double* F_out, double* V_out, skx_slot_status* final_status);
+/* skx_init_field on the lattice in `slot` (add it with F0 = NULL, the vacuum, then initialise it on the
+ * device). */
+int skx_batch_init(skx_batch_handle b, int32_t slot, const skx_init_desc* desc);
 
 /* Diagnostic (no reference counterpart): time `reps` launches of a plain
  * streaming copy of `bytes` bytes on `device`; returns read+write GB/s and

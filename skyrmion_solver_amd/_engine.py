@@ -30,6 +30,9 @@ FP64, FP32 = 0, 1
 FIELD, VELOCITY = 0, 1
 DENSITY_ENERGY, DENSITY_BARYON, DENSITY_POTENTIAL, DENSITY_RHO = 0, 1, 2, 3
 FLAG_NO_GRAPH, FLAG_TRACE = 1, 2
+LUMP_HEDGEHOG, LUMP_RATIONAL_MAP = 0, 1
+PROFILE_EXP, PROFILE_GAUSS, PROFILE_FIT = 0, 1, 2
+MAX_LUMPS = 8
 
 EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
@@ -38,6 +41,7 @@ EXPORTED = (
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
     "skx_batch_remove", "skx_diag_stream_copy", "skx_device_info", "skx_last_error", "skx_version",
+    "skx_init_field", "skx_batch_init",
 )
 
 
@@ -73,6 +77,17 @@ class RunStats(C.Structure):
 class BatchDesc(C.Structure):
     _fields_ = [("theory", C.c_int32), ("precision", C.c_int32), ("device", C.c_int32), ("flags", C.c_uint32),
                 ("tile_z", C.c_int32), ("max_slots", C.c_int32), ("max_blocks", C.c_int32)]
+
+
+class Lump(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("baryon_number", C.c_int32), ("profile", C.c_int32), ("pad_", C.c_int32),
+                ("profile_param", C.c_double), ("center", C.c_double * 3), ("space_rot", C.c_double * 9),
+                ("iso_rot", C.c_double * 9)]
+
+
+class InitDesc(C.Structure):
+    _fields_ = [("n_lumps", C.c_int32), ("vacuum_band", C.c_int32), ("noise", C.c_double),
+                ("noise_seed", C.c_uint64), ("omega_cs", C.c_double), ("lumps", Lump * MAX_LUMPS)]
 
 
 class SlotStatus(C.Structure):
@@ -131,6 +146,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_batch_remove": ([H, C.c_int32, dp, dp, C.POINTER(SlotStatus)], C.c_int),
         "skx_diag_stream_copy": ([C.c_int32, C.c_size_t, C.c_int32, dp, dp], C.c_int),
         "skx_device_info": ([C.c_int32, C.c_char_p, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "skx_init_field": ([H, C.POINTER(InitDesc)], C.c_int),
+        "skx_batch_init": ([H, C.c_int32, C.POINTER(InitDesc)], C.c_int),
         "skx_last_error": ([], C.c_char_p),
         "skx_version": ([], C.c_int),
     }
@@ -228,6 +245,10 @@ class Engine:
     def upload(self, which: int, host: np.ndarray):
         a = np.ascontiguousarray(host, dtype=np.float64).ravel()
         _check(self._lib.skx_upload(self._h, which, _dptr(a), a.size), "skx_upload")
+
+    def init_field(self, desc: "InitDesc"):
+        """Set the field on the device from an ``InitDesc`` (see ``initial_data.device_init_desc``); V = 0."""
+        _check(self._lib.skx_init_field(self._h, C.byref(desc)), "skx_init_field")
 
     def download(self, which: int) -> np.ndarray:
         out = np.empty(self.nf * self.N, dtype=np.float64)
@@ -329,18 +350,29 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
-    def add(self, p_i, p_f, F0, V0=None, max_steps: int = 100000, tol: float = -1.0, device: int = 0,
-            x_chunk: int = 0) -> int:
+    def add(self, p_i, p_f, F0=None, V0=None, max_steps: int = 100000, tol: float = -1.0, device: int = 0,
+            x_chunk: int = 0, init: "InitDesc | None" = None) -> int:
         """Add a lattice; raises ``SkxError`` (code -5, SKX_EBUSY) when the batch is full.
 
-        ``x_chunk`` = planes per block along x (0: the whole lattice)."""
+        The field is `F0` (host FP64), or built on the device from `init`
+        (an ``InitDesc``), or the vacuum when both are None.  ``x_chunk`` =
+        planes per block along x (0: the whole lattice)."""
         d, keep = make_desc(self.theory, self.precision, p_i, p_f, device, 0, 0, x_chunk)
-        f = np.ascontiguousarray(F0, dtype=np.float64).ravel()
+        pi = np.asarray(p_i)
+        n = int(pi[5]) * int(pi[0]) * int(pi[1]) * int(pi[2])
+        f = None if F0 is None else np.ascontiguousarray(F0, dtype=np.float64).ravel()
         v = None if V0 is None else np.ascontiguousarray(V0, dtype=np.float64).ravel()
         slot = C.c_int32()
-        _check(self._lib.skx_batch_add(self._h, C.byref(d), _dptr(f), _dptr(v) if v is not None else None,
-                                       int(max_steps), float(tol), C.byref(slot)), "skx_batch_add")
-        self._shape[slot.value] = f.size
+        _check(self._lib.skx_batch_add(self._h, C.byref(d), _dptr(f) if f is not None else None,
+                                       _dptr(v) if v is not None else None, int(max_steps), float(tol),
+                                       C.byref(slot)), "skx_batch_add")
+        self._shape[slot.value] = n
+        if init is not None:
+            try:
+                _check(self._lib.skx_batch_init(self._h, slot.value, C.byref(init)), "skx_batch_init")
+            except SkxError:
+                self.remove(slot.value, want_fields=False)
+                raise
         return slot.value
 
     def run(self, steps: int):

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "../../include/skx.h"
 #include "skx_physics.h"
 
 namespace skx {
@@ -114,6 +115,17 @@ hipError_t stage_launch_batch(int stage, const LaunchShape& s, const BatchSlot<T
                               int parity, int grid, hipStream_t st);
 template <typename T, int TH>
 hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks);
+
+// Device initial data (skx_init.hip).
+struct InitArgs {
+  skx_init_desc d;
+  int X, Y, Z, nf, halo;
+  double lsx, lsy, lsz;
+};
+hipError_t launch_init_field_f64(double* F, double* V, const InitArgs& a, hipStream_t st);
+hipError_t launch_init_field_f32(float* F, float* V, const InitArgs& a, hipStream_t st);
+hipError_t launch_set_component_f64(double* Fc, const double* dens, long long N, double scale, hipStream_t st);
+hipError_t launch_set_component_f32(float* Fc, const double* dens, long long N, double scale, hipStream_t st);
 
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);  // zero fill kernel (graph safe)
 hipError_t launch_apply_arrest_f64(double* V, long long n, const DevStatus* st, hipStream_t s);

@@ -411,6 +411,54 @@ int ensure_dens2(skx_engine* e) {
   return SKX_OK;
 }
 
+// Device initial data (skx_init_field): the ansatz into F[cur], V[cur] = 0,
+// and for the omega theory omega = -c CS from the stage-0 baryon density.
+int init_field(skx_engine* e, const skx_init_desc* d) {
+  if (!d) return fail(SKX_EINVAL, "null init descriptor");
+  if (d->n_lumps < 0 || d->n_lumps > SKX_MAX_LUMPS) return fail(SKX_EINVAL, "n_lumps must be 0..SKX_MAX_LUMPS");
+  for (int l = 0; l < d->n_lumps; ++l) {
+    const skx_lump& q = d->lumps[l];
+    if (q.kind != SKX_LUMP_HEDGEHOG && q.kind != SKX_LUMP_RATIONAL_MAP) return fail(SKX_EINVAL, "unknown lump kind");
+    if (q.profile != SKX_PROFILE_EXP && q.profile != SKX_PROFILE_GAUSS && q.profile != SKX_PROFILE_FIT)
+      return fail(SKX_EINVAL, "unknown profile kind");
+    if (q.profile != SKX_PROFILE_FIT && !(q.profile_param > 0.0))
+      return fail(SKX_EINVAL, "exp / gauss profile length must be positive");
+  }
+  if (!(d->noise >= 0.0)) return fail(SKX_EINVAL, "noise must be >= 0");
+  if (d->omega_cs != 0.0 && e->L.theory != SKX_THEORY_OMEGA)
+    return fail(SKX_EINVAL, "omega_cs applies to the omega theory only");
+  const Lattice& L = e->L;
+  InitArgs a{};
+  a.d = *d;
+  a.X = L.X;
+  a.Y = L.Y;
+  a.Z = L.Z;
+  a.nf = L.nf;
+  a.halo = L.halo;
+  a.lsx = L.lsx;
+  a.lsy = L.lsy;
+  a.lsz = L.lsz;
+  hipError_t r = L.prec == SKX_FP64
+                     ? launch_init_field_f64(static_cast<double*>(e->F[e->cur]), static_cast<double*>(e->V[e->cur]), a,
+                                             e->stream)
+                     : launch_init_field_f32(static_cast<float*>(e->F[e->cur]), static_cast<float*>(e->V[e->cur]), a,
+                                             e->stream);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("init kernel: ") + hipGetErrorString(r));
+  if (d->omega_cs != 0.0) {
+    if (int rc = ensure_dens2(e)) return rc;
+    r = launch(e, 0, FUSE_BARYON, e->cur, nullptr, nullptr, e->dens2);  // CS * vol per site
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("Chern-Simons pass: ") + hipGetErrorString(r));
+    const double scale = -d->omega_cs / L.P.vol;
+    const size_t off = (size_t)4 * L.N;
+    r = L.prec == SKX_FP64
+            ? launch_set_component_f64(static_cast<double*>(e->F[e->cur]) + off, e->dens2, L.N, scale, e->stream)
+            : launch_set_component_f32(static_cast<float*>(e->F[e->cur]) + off, e->dens2, L.N, scale, e->stream);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("omega init: ") + hipGetErrorString(r));
+  }
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return SKX_OK;
+}
+
 int copy_in(skx_engine* e, void* dst, const double* host, size_t n) {
   if (e->L.prec == SKX_FP64) {
     HIP_TRY(hipMemcpy(dst, host, n * sizeof(double), hipMemcpyHostToDevice));
@@ -687,6 +735,11 @@ int skx_density(skx_handle h, int32_t kind, double* host, size_t n) {
     return SKX_OK;
   }
   return fail(SKX_EINVAL, "unknown density kind");
+}
+
+int skx_init_field(skx_handle h, const skx_init_desc* desc) {
+  CHECK_HANDLE(h);
+  return init_field(h, desc);
 }
 
 int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n) {
@@ -967,7 +1020,7 @@ int skx_batch_destroy(skx_batch_handle b) {
 int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0, const double* V0, int32_t max_steps,
                   double tol, int32_t* slot_out) {
   CHECK_BATCH(b);
-  if (!lattice || !F0 || !slot_out) return fail(SKX_EINVAL, "null argument");
+  if (!lattice || !slot_out) return fail(SKX_EINVAL, "null argument");
   if (lattice->theory != b->theory || lattice->precision != b->prec)
     return fail(SKX_EINVAL, "lattice theory / precision differ from the batch's");
   if (max_steps < 0) return fail(SKX_EINVAL, "max_steps must be >= 0");
@@ -995,9 +1048,17 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   skx_engine* e = nullptr;
   if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, xc, &e)) return rc;
   const size_t n = (size_t)L.N * L.nf;
-  if (int rc = copy_in(e, e->F[0], F0, n)) {
-    release(e);
-    return rc;
+  if (F0) {
+    if (int rc = copy_in(e, e->F[0], F0, n)) {
+      release(e);
+      return rc;
+    }
+  } else {  // the vacuum (skx_batch_init may set a configuration on the device)
+    skx_init_desc vac{};
+    if (int rc = init_field(e, &vac)) {
+      release(e);
+      return rc;
+    }
   }
   if (V0) {
     if (int rc = copy_in(e, e->V[0], V0, n)) {
@@ -1062,6 +1123,14 @@ int skx_batch_poll(skx_batch_handle b, skx_slot_status* out, int32_t n, int32_t*
     if (b->slot[i]) slot_status(b, i, &out[k++]);
   *n_out = k;
   return SKX_OK;
+}
+
+int skx_batch_init(skx_batch_handle b, int32_t slot, const skx_init_desc* desc) {
+  CHECK_BATCH(b);
+  if (slot < 0 || slot >= b->max_slots || !b->slot[slot]) return fail(SKX_EINVAL, "no lattice in that slot");
+  if (b->h_status[slot].steps != 0) return fail(SKX_EINVAL, "the lattice has already run; initialise it right after skx_batch_add");
+  // lattices of a batch are at ping-pong parity 0 between chunks
+  return init_field(b->slot[slot], desc);
 }
 
 int skx_batch_remove(skx_batch_handle b, int32_t slot, double* F_out, double* V_out, skx_slot_status* final_status) {

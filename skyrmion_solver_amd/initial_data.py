@@ -323,8 +323,47 @@ def chern_simons_density(F: np.ndarray, rp) -> np.ndarray:
     return -det / (2.0 * np.pi * np.pi)
 
 
+def _mode(rp, cfg) -> str:
+    mode = str(cfg.get("mode", "initial")).lower().strip()
+    if mode == "initial":
+        if getattr(rp, "ansatz_rational_map", False):
+            mode = "rational_map"
+        elif getattr(rp, "ansatz_uniform", False):
+            mode = "ground"
+        else:
+            mode = "hedgehog"
+    if mode in ("rational map", "rational-map"):
+        mode = "rational_map"
+    return mode
+
+
+def lump_specs(rp, config: dict | None = None):
+    """The lumps an ``initialize(config)`` request builds, with every random draw made.
+
+    Returns ``(lumps, extra)``: ``lumps`` is a list of dicts (``mode``,
+    ``B``, ``iso``, ``profile``, ``center``, ``rotation``, ``space_rotation``)
+    whose quaternion product is the field; ``extra`` holds ``noise``,
+    ``seed`` and ``vacuum_band``.  :func:`build_field` evaluates them on the
+    host and :func:`device_init_desc` hands them to the device."""
+    cfg = dict(config or {})
+    rng = np.random.default_rng(cfg.get("seed", 0))
+    mode = _mode(rp, cfg)
+    if mode in ("ground", "uniform", "vacuum", "gs"):
+        lumps = []
+    elif mode in ("hedgehog", "rational_map"):
+        lumps = [_lump_spec(rp, mode, cfg, rng)]
+    elif mode == "product":
+        lumps = [_lump_spec(rp, _mode(rp, dict(lump, mode=lump.get("mode", "rational_map"))), lump, rng)
+                 for lump in cfg.get("lumps", [])]
+    else:
+        raise ValueError(f"unknown initial-data mode {mode!r}")
+    extra = dict(noise=float(cfg.get("noise", 0.0)), seed=int(cfg.get("seed", 0)),
+                 vacuum_band=bool(cfg.get("vacuum_band", False)), rng=rng)
+    return lumps, extra
+
+
 def build_field(rp, nf: int, config: dict | None = None) -> np.ndarray:
-    """Initial field ``(nf, X, Y, Z)`` for an ``initialize(config)`` request.
+    """Initial field ``(nf, X, Y, Z)`` for an ``initialize(config)`` request, built on the host.
 
     ``mode``: ``ground`` / ``uniform`` / ``vacuum``; ``hedgehog``;
     ``rational_map``; ``initial`` (the ansatz named by the parameters);
@@ -336,40 +375,23 @@ def build_field(rp, nf: int, config: dict | None = None) -> np.ndarray:
     (``"haar"``), ``noise`` (N(0, noise) per component before
     renormalisation), ``seed``, ``vacuum_band`` (bool).
     """
-    cfg = dict(config or {})
-    rng = np.random.default_rng(cfg.get("seed", 0))
-    mode = str(cfg.get("mode", "initial")).lower().strip()
-    if mode == "initial":
-        if getattr(rp, "ansatz_rational_map", False):
-            mode = "rational_map"
-        elif getattr(rp, "ansatz_uniform", False):
-            mode = "ground"
-        else:
-            mode = "hedgehog"
+    lumps, extra = lump_specs(rp, config)
     F = vacuum(rp, nf)
-    if mode in ("ground", "uniform", "vacuum", "gs"):
-        pass
-    elif mode in ("hedgehog", "rational_map", "rational map", "rational-map"):
-        F[:4] = _lump(rp, mode, cfg, rng)
-    elif mode == "product":
-        acc = None
-        for lump in cfg.get("lumps", []):
-            part = _lump(rp, str(lump.get("mode", "rational_map")), lump, rng)
-            acc = part if acc is None else normalise_sigma(quaternion_product(acc, part))
-        if acc is not None:
-            F[:4] = acc
-    else:
-        raise ValueError(f"unknown initial-data mode {mode!r}")
-    noise = float(cfg.get("noise", 0.0))
-    if noise > 0.0:
-        F[:4] += rng.normal(0.0, noise, size=F[:4].shape)
+    acc = None
+    for spec in lumps:
+        part = _eval_lump(rp, spec)
+        acc = part if acc is None else normalise_sigma(quaternion_product(acc, part))
+    if acc is not None:
+        F[:4] = acc
+    if extra["noise"] > 0.0:
+        F[:4] += extra["rng"].normal(0.0, extra["noise"], size=F[:4].shape)
     normalise_sigma(F)
-    if cfg.get("vacuum_band", False):
+    if extra["vacuum_band"]:
         set_vacuum_band(F, int(getattr(rp, "halo", 2)))
     return F
 
 
-def _lump(rp, mode, cfg, rng):
+def _lump_spec(rp, mode, cfg, rng):
     B = float(cfg.get("baryon_number", getattr(rp, "baryon_number", 1.0)))
     iso = float(cfg.get("isospin_rotation", getattr(rp, "isospin_rotation", 0.0)))
     center = np.array(cfg.get("center", (0.0, 0.0, 0.0)), dtype=np.float64)
@@ -378,7 +400,73 @@ def _lump(rp, mode, cfg, rng):
         center = center + rng.uniform(-jitter, jitter, size=3)
     rot = haar_rotation(rng) if cfg.get("isorotation") == "haar" else None
     srot = haar_rotation(rng) if cfg.get("space_rotation") == "haar" else None
-    profile = cfg.get("profile")
-    if mode == "hedgehog":
-        return hedgehog(rp, B, iso, profile, center, rotation=rot)
-    return rational_map_skyrmion(rp, B, iso, profile, center, rotation=rot, space_rotation=srot)
+    return dict(mode=mode, B=B, iso=iso, profile=cfg.get("profile"), center=center, rotation=rot,
+                space_rotation=srot)
+
+
+def _eval_lump(rp, spec):
+    if spec["mode"] == "hedgehog":
+        return hedgehog(rp, spec["B"], spec["iso"], spec["profile"], spec["center"], rotation=spec["rotation"])
+    return rational_map_skyrmion(rp, spec["B"], spec["iso"], spec["profile"], spec["center"],
+                                 rotation=spec["rotation"], space_rotation=spec["space_rotation"])
+
+
+def _apply(fn, *args):
+    """3x3 matrix of a linear map given as a function of three coordinates."""
+    M = np.zeros((3, 3))
+    for j in range(3):
+        e = [0.0, 0.0, 0.0]
+        e[j] = 1.0
+        M[:, j] = fn(*e, *args)
+    return M
+
+
+def device_init_desc(rp, config: dict | None = None, omega_cs: float = 0.0):
+    """The ``skx_init_desc`` of an ``initialize(config)`` request (the same lumps as :func:`build_field`).
+
+    The device adds noise from its own counter-based generator keyed by the
+    config's ``seed``, not NumPy's stream: with ``noise > 0`` the device and
+    host fields differ by the noise realisation only."""
+    from skyrmion_solver_amd import _engine
+
+    lumps, extra = lump_specs(rp, config)
+    if len(lumps) > _engine.MAX_LUMPS:
+        raise ValueError(f"at most {_engine.MAX_LUMPS} lumps")
+    d = _engine.InitDesc()
+    d.n_lumps = len(lumps)
+    d.vacuum_band = int(extra["vacuum_band"])
+    d.noise = extra["noise"]
+    d.noise_seed = extra["seed"] & ((1 << 64) - 1)
+    d.omega_cs = float(omega_cs)
+    for k, sp in enumerate(lumps):
+        q = d.lumps[k]
+        q.kind = _engine.LUMP_HEDGEHOG if sp["mode"] == "hedgehog" else _engine.LUMP_RATIONAL_MAP
+        q.baryon_number = max(int(abs(sp["B"]) + 0.5), 1)
+        prof = sp["profile"]
+        if prof is None and sp["mode"] == "hedgehog":
+            rmax = max(rp.xsize, rp.ysize, rp.zsize)
+            prof = ("gauss", max(rmax / (8.0 * max(abs(sp["B"]), 1.0)), 1.0e-6))
+        kind = prof[0] if isinstance(prof, (tuple, list)) else prof
+        if prof is None or kind == "fit":
+            q.profile, q.profile_param = _engine.PROFILE_FIT, 1.0
+        elif kind == "exp":
+            q.profile, q.profile_param = _engine.PROFILE_EXP, float(prof[1])
+        elif kind == "gauss":
+            q.profile, q.profile_param = _engine.PROFILE_GAUSS, float(prof[1])
+        else:
+            raise ValueError(f"unknown profile {prof!r}")
+        for i in range(3):
+            q.center[i] = float(sp["center"][i])
+        S = np.eye(3) if sp["space_rotation"] is None or sp["mode"] == "hedgehog" else np.asarray(sp["space_rotation"])
+        if sp["rotation"] is not None:
+            R = np.asarray(sp["rotation"])
+        elif sp["mode"] == "hedgehog":
+            R = _apply(rotate_by_z, sp["iso"])
+        else:
+            R = _apply(rotate_xyz, 0.0, 0.0, sp["iso"])
+        for i in range(9):
+            q.space_rot[i] = float(S.flat[i])
+            q.iso_rot[i] = float(R.flat[i])
+    return d
+
+

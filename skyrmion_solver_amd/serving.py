@@ -46,6 +46,7 @@ class Request:
     max_steps: int = 20000
     arrival: float = 0.0
     V0: np.ndarray | None = None
+    init: object = None  # _engine.InitDesc: build the field on the device at admission (F0 None)
     # filled in by the server
     admitted: float | None = None
     finished: float | None = None
@@ -108,7 +109,7 @@ class Server:
                 return
             try:
                 slot = self._batch(req.theory).add(req.p_i, req.p_f, req.F0, req.V0, max_steps=req.max_steps,
-                                                   tol=req.tol, device=self.device)
+                                                   tol=req.tol, device=self.device, init=req.init)
             except _engine.SkxError as e:
                 if e.code == -5:  # SKX_EBUSY: no slot / blocks free in this batch now
                     return
@@ -160,13 +161,27 @@ class Server:
         return self.report()
 
     def report(self) -> dict:
+        """Time-to-result (arrival -> result) p50 / p99 over all finished requests and over the converged
+        ones, throughput, and per-theory counts (converged, stopped at max_steps, non-finite)."""
         lat = np.array([r.latency for r in self.done]) if self.done else np.zeros(1)
+        conv = np.array([r.latency for r in self.done if r.status.get("converged")])
         span = max((r.finished for r in self.done), default=0.0) - min((r.arrival for r in self.done), default=0.0)
+        by = {}
+        for r in self.done:
+            t = by.setdefault(r.theory, dict(requests=0, converged=0, nonfinite=0, steps=[]))
+            t["requests"] += 1
+            t["converged"] += int(bool(r.status.get("converged")))
+            t["nonfinite"] += int(bool(r.status.get("nonfinite")))
+            t["steps"].append(int(r.status.get("steps", 0)))
+        for t in by.values():
+            t["median_steps"] = int(np.median(t.pop("steps")))
         return dict(requests=len(self.done), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                    mean_s=float(lat.mean()), makespan_s=float(span),
-                    site_updates_per_s=self.site_updates / span if span > 0 else 0.0,
-                    converged=sum(int(r.status.get("converged", 0)) for r in self.done),
-                    peak_hbm_bytes=self.peak_used)
+                    mean_s=float(lat.mean()),
+                    converged_p50_s=float(np.percentile(conv, 50)) if conv.size else None,
+                    converged_p99_s=float(np.percentile(conv, 99)) if conv.size else None,
+                    makespan_s=float(span), site_updates_per_s=self.site_updates / span if span > 0 else 0.0,
+                    converged=int(sum(int(bool(r.status.get("converged"))) for r in self.done)),
+                    by_theory=by, peak_hbm_bytes=self.peak_used)
 
     def close(self):
         for b in self.batches.values():
@@ -178,3 +193,37 @@ def poisson_arrivals(n: int, rate: float, seed: int = 0) -> np.ndarray:
     """Arrival times (s) of a Poisson stream of `rate` requests per second."""
     rng = np.random.default_rng(seed)
     return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def main(argv=None):
+    """Serving sweep: Poisson request streams of the theory mix at several rates, one JSON line per rate."""
+    import argparse
+    import json
+
+    from skyrmion_solver_amd import workloads
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--rates", type=float, nargs="+", default=[5, 10, 20, 50, 100, 200])
+    ap.add_argument("--n", type=int, default=600, help="requests per rate")
+    ap.add_argument("--sizes", type=int, nargs="+", default=[64, 96, 128, 160, 192])
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--tol", type=float, default=1.0e-4)
+    ap.add_argument("--max-steps", type=int, default=20000)
+    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--hbm-gib", type=float, default=200.0)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    prec = _engine.FP32 if args.precision == "fp32" else _engine.FP64
+    for rate in args.rates:
+        reqs = workloads.c5_requests(n=args.n, rate=rate, seed=args.seed, sizes=args.sizes, tol=args.tol,
+                                     max_steps=args.max_steps, device_init=True)
+        srv = Server(precision=prec, hbm_budget_bytes=int(args.hbm_gib * (1 << 30)), chunk_steps=args.chunk)
+        rep = srv.serve(reqs)
+        srv.close()
+        rep.update(rate_per_s=rate, precision=args.precision, sizes=args.sizes, tol=args.tol,
+                   max_steps=args.max_steps)
+        print(json.dumps(rep), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -68,9 +68,10 @@ def c3_b8(theory: str = "omega", seed: int = 0, L: int = 160):
     return th, rp, p_i, p_f, F
 
 
-def c4_instances(n: int = 48, seed: int = 0, sizes=(48, 64, 96, 128), dx: float = 0.15):
+def c4_instances(n: int = 48, seed: int = 0, sizes=(48, 64, 96, 128), dx: float = 0.15, device_init: bool = False):
     """Independent pion lattices: L from `sizes`, B = 1..6 rational maps with random space and
-    isospin orientations, m ~ U[0, 1.5]."""
+    isospin orientations, m ~ U[0, 1.5].  Each entry carries the host field ``F``, or with
+    `device_init` the ``init`` descriptor that builds the same field on the device."""
     rng = np.random.default_rng(seed)
     th = load_theory("Nuclear Skyrme model")
     out = []
@@ -80,10 +81,12 @@ def c4_instances(n: int = 48, seed: int = 0, sizes=(48, 64, 96, 128), dx: float 
         m = float(rng.uniform(0.0, 1.5))
         rp = th.params.default_params(**_lattice(L, dx), mpi=m, kappa=1.0, baryon_number=float(B)).resolved()
         p_i, p_f = th.params.pack_device_params(rp)
-        F = initial_data.build_field(rp, 4, dict(mode="rational_map", baryon_number=B, profile=("exp", 1.0 + 0.15 * B),
-                                                isorotation="haar", space_rotation="haar",
-                                                seed=int(rng.integers(1 << 30)), vacuum_band=True))
-        out.append(dict(L=L, B=B, m=m, rp=rp, p_i=p_i, p_f=p_f, F=F))
+        cfg = dict(mode="rational_map", baryon_number=B, profile=("exp", 1.0 + 0.15 * B), isorotation="haar",
+                   space_rotation="haar", seed=int(rng.integers(1 << 30)), vacuum_band=True)
+        if device_init:
+            out.append(dict(L=L, B=B, m=m, rp=rp, p_i=p_i, p_f=p_f, F=None, init=initial_data.device_init_desc(rp, cfg)))
+        else:
+            out.append(dict(L=L, B=B, m=m, rp=rp, p_i=p_i, p_f=p_f, F=initial_data.build_field(rp, 4, cfg), init=None))
     return out
 
 
@@ -101,15 +104,24 @@ def set_omega_from_chern_simons(engine, rp):
 
 
 def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0e-4, max_steps: int = 20000,
-                    arrival: float = 0.0):
-    """One minimisation request: a B = 1..8 rational map of random space / isospin orientation on an L^3
-    lattice, in each theory's working units (the reference's examples):
+                    arrival: float = 0.0, device_init: bool = False):
+    """One minimisation request: a B = 1..8 rational map (random isospin orientation, aligned with the
+    lattice) on an L^3 lattice with seeded parameters:
 
     * pion: dx = 0.15, courant 0.2, m ~ U[0.5, 1.5], kappa = 1;
-    * omega: dx = 12/63, courant 0.2, f_pi = 139.8, m_pi = 43.91, m_omega = 249.5, beta_omega ~ U[7.5, 8.5]
-      (c_omega ~ 14, examples/omega_skyrme_gl.py:11-16), omega started at -c_omega CS;
-    * rho: dx = 2/63, courant 0.05, e = 3.65, f_pi = 129, m_pi = 138, m_rho = 775, alpha ~ U[0.055, 0.068]
-      (examples/rho_skyrme_gl.py:9-15), rho started at 0.
+    * omega: the units of examples/omega_skyrme_gl.py:11-16 (dx = 12/63, courant 0.2, f_pi = 139.8,
+      m_pi = 43.91, m_omega = 249.5) with beta_omega ~ U[7.5, 8.5] (c_omega ~ 14); omega starts at
+      -c_omega CS;
+    * rho: dx = 0.15, courant 0.1 (the rho Laplacian has weight 8), the physical masses of
+      examples/rho_skyrme_gl.py:12-13 (e = 3.65, f_pi = 129, m_pi = 138, m_rho = 775) with
+      alpha ~ U[0.055, 0.068]; rho starts at 0.
+
+    With `device_init` the request carries the descriptor that builds its
+    field on the device at admission (``Request.init``) instead of a host
+    field.
+
+    Space orientation stays aligned: a misaligned B >= 4 solution relaxes through a slow lattice
+    rotation mode, which is physics of the discretisation rather than of the server.
     """
     from skyrmion_solver_amd.serving import Request
 
@@ -121,7 +133,7 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
         lat = _lattice(L, 12.0 / 63.0, 0.2)
         extra = dict(f_pi=139.8, m_pi=43.91, m_omega=249.5, beta_omega=float(rng.uniform(7.5, 8.5)))
     elif theory == "rho":
-        lat = _lattice(L, 2.0 / 63.0, 0.05)
+        lat = _lattice(L, 0.15, 0.1)
         extra = dict(e_skyrme=3.65, f_pi=129.0, mass_pion=138.0, mass_rho=775.0, alpha=float(rng.uniform(0.055, 0.068)))
     else:
         lat = _lattice(L, 0.15, 0.2)
@@ -129,19 +141,20 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
     rp = th.params.default_params(**lat, baryon_number=float(B), **extra).resolved()
     p_i, p_f = th.params.pack_device_params(rp)
     nf = int(p_i[5])
-    # profile length: the pion / omega units hold a soliton of radius ~1; the rho box is 2 units wide, so its
-    # profile scales with the box (as the reference's hedgehog does, r0 = rmax / (8 B))
-    a = (1.0 + 0.15 * B) if theory != "rho" else float(rp.xsize) * (1.0 + 0.15 * B) / 14.0
-    F = initial_data.build_field(rp, nf, dict(mode="rational_map", baryon_number=B, profile=("exp", a),
-                                             isorotation="haar", space_rotation="haar",
-                                             seed=int(rng.integers(1 << 30)), vacuum_band=True))
+    cfg = dict(mode="rational_map", baryon_number=B, profile=("exp", 1.0 + 0.15 * B), isorotation="haar",
+               seed=int(rng.integers(1 << 30)), vacuum_band=True)
+    c_cs = float(rp.c_omega) if theory == "omega" else 0.0
+    if device_init:
+        return Request(rid=rid, theory=theory, p_i=p_i, p_f=p_f, F0=None, tol=tol, max_steps=max_steps,
+                       arrival=arrival, init=initial_data.device_init_desc(rp, cfg, omega_cs=c_cs))
+    F = initial_data.build_field(rp, nf, cfg)
     if theory == "omega":
-        F[4] = -float(rp.c_omega) * initial_data.chern_simons_density(F, rp)
+        F[4] = -c_cs * initial_data.chern_simons_density(F, rp)
     return Request(rid=rid, theory=theory, p_i=p_i, p_f=p_f, F0=F, tol=tol, max_steps=max_steps, arrival=arrival)
 
 
 def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 128, 160, 192), mix=SERVE_MIX,
-                tol: float = 1.0e-4, max_steps: int = 20000):
+                tol: float = 1.0e-4, max_steps: int = 20000, device_init: bool = False):
     """Poisson stream of `n` requests at `rate` per second: theory drawn from `mix`, L from `sizes`, B = 1..8."""
     from skyrmion_solver_amd.serving import poisson_arrivals
 
@@ -155,5 +168,6 @@ def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 
         theory = str(rng.choice(names, p=probs))
         L = int(rng.choice(sizes))
         B = int(rng.integers(1, 9))
-        out.append(serving_request(k, theory, L, B, rng, tol=tol, max_steps=max_steps, arrival=float(arrivals[k])))
+        out.append(serving_request(k, theory, L, B, rng, tol=tol, max_steps=max_steps, arrival=float(arrivals[k]),
+                                   device_init=device_init))
     return out

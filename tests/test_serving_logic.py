@@ -17,7 +17,7 @@ class FakeBatch:
         self.theory, self.slots, self.log = theory, slots, log
         self.lat = {}
 
-    def add(self, p_i, p_f, F0, V0, max_steps, tol, device):
+    def add(self, p_i, p_f, F0, V0, max_steps, tol, device, init=None):
         if len(self.lat) >= self.slots:
             raise _engine.SkxError(-5, "skx_batch_add", "busy")
         s = min(set(range(self.slots)) - set(self.lat))
