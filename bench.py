@@ -197,14 +197,14 @@ def main():
     ap.add_argument("--solve-seeds", type=int, default=3)
     ap.add_argument("--solve-max-steps", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1.0e-4)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps of the CPU baseline (~4.6 s each)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-n", type=int, default=48, help="lattices of the batched sample (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=40)
     ap.add_argument("--serve-n", type=int, default=24, help="requests of the serving sample (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-sizes", type=int, nargs="+", default=[48, 64, 96])
-    ap.add_argument("--serve-max-steps", type=int, default=6000)
+    ap.add_argument("--serve-sizes", type=int, nargs="+", default=[64, 96])
+    ap.add_argument("--serve-max-steps", type=int, default=8000)
     ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
                     help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
     args = ap.parse_args()

@@ -74,3 +74,22 @@ def test_c3_b8_fp32_form_against_fp64(theory):
     assert abs(s32.energy - s64.energy) <= 1e-5 * abs(s64.energy)
     assert abs(s32.baryon - s64.baryon) <= 1e-4
     assert rel_l2(F32, F64) <= 1e-4
+
+
+def test_fp32_form_converges_to_the_fp64_solution():
+    """96^3 pion B=4 (aligned rational map, m = 1): both forms relaxed to max|g| < 1e-4; the FP32 solution's
+    energy within 1e-6 relative, B within 1e-4 and field within 1e-4 relative L2 of the FP64 solution."""
+    print("[case] 96^3 B=4 fp32 vs fp64 to convergence", flush=True)
+    th, rp, p_i, p_f, F = workloads.c2_b4(seed=1, L=96)
+    out = {}
+    for prec in (_engine.FP64, _engine.FP32):
+        with _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f) as eng:
+            eng.upload(_engine.FIELD, F)
+            st = eng.anf_run(20000, 1.0e-4, 64)
+            out[prec] = (st, eng.download(_engine.FIELD))
+    (s64, F64), (s32, F32) = out[_engine.FP64], out[_engine.FP32]
+    print("steps fp64/fp32:", s64.steps, s32.steps, "E:", s64.energy, s32.energy, "B:", s64.baryon, s32.baryon)
+    assert s64.converged == 1 and s32.converged == 1
+    assert abs(s32.energy - s64.energy) <= 1e-6 * abs(s64.energy)
+    assert abs(s32.baryon - s64.baryon) <= 1e-4
+    assert rel_l2(F32, F64) <= 1e-4
