@@ -147,6 +147,35 @@ int skx_density(skx_handle h, int32_t kind, double* host, size_t n);
  * (either may be NULL). */
 int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n);
 
+/* ---------------------------------------------------------------- observables
+ * Post-convergence observables: one FP64 pass over the lattice evaluates the
+ * per-site integrand the reference's observable kernel writes and returns its
+ * lattice sums (the reference reduces the same per-site arrays,
+ * theories/nuclear_skyrme/observables.py:182-486):
+ *   COM         4: sum b, sum x b, sum y b, sum z b   compute_center_of_mass_kernel (kernels.py:678-718)
+ *   RMS         1: sum |r|^2 |b|                      compute_rms_radius_kernel (:720-759)
+ *   QUADRUPOLE  9: sum (3 r_i r_j - d_ij r^2) b / 2   compute_quadrupole_tensor_kernel (:761-812)
+ *   U           9: sum 2 U0_ij vol                    compute_U_tensor_kernel (:814-851)
+ *   V           9: sum 2 e_ilm e_jnp r_l r_n A_mp vol compute_V_tensor_kernel (:853-905)
+ *   W           9: sum 2 e_jlm r_l B_im vol           compute_W_tensor_kernel (:907-975)
+ *   D           1: sum (r_i r_j T_ij - r^2 T_kk / 3) vol  compute_D_integrand_kernel (:977-1033)
+ * b is the baryon density times the cell volume, the grid is x = lsx * i
+ * (create_grid_kernel, kernels.py:50-56) and r = x - com; tensors are
+ * row-major (3 i + j).  `com` (3 doubles) is read by RMS, QUADRUPOLE, V, W
+ * and D only.  COM and RMS exist for every theory (e.g.
+ * theories/omega_skyrme/kernels.py:503,545); the others for the nuclear
+ * theory only (SKX_EINVAL otherwise).  n must be at least the count above. */
+enum skx_observable_kind {
+  SKX_OBS_COM = 0,
+  SKX_OBS_RMS = 1,
+  SKX_OBS_QUADRUPOLE = 2,
+  SKX_OBS_U = 3,
+  SKX_OBS_V = 4,
+  SKX_OBS_W = 5,
+  SKX_OBS_D = 6
+};
+int skx_observable(skx_handle h, int32_t kind, const double* com, double* out, int32_t n);
+
 /* ---------------------------------------------------------------- initial data
  * Device-side initial configurations.  Replaces the host-built field of
  * Simulation.initialize -> theory initialize (core/simulation.py:124-173) with

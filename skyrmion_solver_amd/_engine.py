@@ -33,6 +33,8 @@ FLAG_NO_GRAPH, FLAG_TRACE = 1, 2
 LUMP_HEDGEHOG, LUMP_RATIONAL_MAP = 0, 1
 PROFILE_EXP, PROFILE_GAUSS, PROFILE_FIT = 0, 1, 2
 MAX_LUMPS = 8
+OBS_COM, OBS_RMS, OBS_QUADRUPOLE, OBS_U, OBS_V, OBS_W, OBS_D = 0, 1, 2, 3, 4, 5, 6
+OBS_COUNT = (4, 1, 9, 9, 9, 9, 1)
 
 EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
@@ -41,7 +43,7 @@ EXPORTED = (
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
     "skx_batch_remove", "skx_diag_stream_copy", "skx_device_info", "skx_last_error", "skx_version",
-    "skx_init_field", "skx_batch_init",
+    "skx_init_field", "skx_batch_init", "skx_observable",
 )
 
 
@@ -130,6 +132,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_energy": ([H, dp], C.c_int),
         "skx_baryon": ([H, dp], C.c_int),
         "skx_density": ([H, C.c_int32, dp, C.c_size_t], C.c_int),
+        "skx_observable": ([H, C.c_int32, dp, dp, C.c_int32], C.c_int),
         "skx_eval_gradient": ([H, dp, dp, dp, C.c_size_t], C.c_int),
         "skx_history_attach": ([H, C.c_int32], C.c_int),
         "skx_history_read": ([H, C.POINTER(StepRecord), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
@@ -287,6 +290,15 @@ class Engine:
     def density(self, kind: int) -> np.ndarray:
         out = np.empty(self.N, dtype=np.float64)
         _check(self._lib.skx_density(self._h, int(kind), _dptr(out), out.size), "skx_density")
+        return out
+
+    def observable(self, kind: int, com=None) -> np.ndarray:
+        """Lattice sums of one observable's per-site integrand (``OBS_*``; FP64)."""
+        out = np.empty(OBS_COUNT[kind], dtype=np.float64)
+        c = np.ascontiguousarray(np.zeros(3) if com is None else com, dtype=np.float64)
+        if c.size != 3:
+            raise ValueError("com must hold 3 values")
+        _check(self._lib.skx_observable(self._h, int(kind), _dptr(c), _dptr(out), out.size), "skx_observable")
         return out
 
     def eval_gradient(self, temp: np.ndarray):

@@ -155,6 +155,38 @@ class Simulation:
         """Sum of squared rho components into ``en`` (``:270-285``)."""
         self.en = HostDensity(self.engine.density(_engine.DENSITY_RHO))
 
+    # ------------------------------------------------ post-convergence observables
+    # core/simulation.py:390-493: each delegates to the theory's observables
+    # module, which raises AttributeError where the theory has no such observable
+    # (the omega / Coulomb / rho theories have the centre of mass and RMS radius).
+    def compute_center_of_mass(self) -> np.ndarray:
+        """Baryon-density centre of mass, shape ``(3,)`` (``core/simulation.py:390-403``)."""
+        return self.observables_mod.compute_center_of_mass(self)
+
+    def compute_rms_radius(self) -> float:
+        """RMS radius about the centre of mass (``:405-418``)."""
+        return self.observables_mod.compute_rms_radius(self)
+
+    def compute_U_tensor(self) -> np.ndarray:
+        """Isospin inertia tensor ``U``, shape ``(3, 3)`` (``:420-433``)."""
+        return self.observables_mod.compute_U_tensor(self)
+
+    def compute_V_tensor(self) -> np.ndarray:
+        """Spin inertia tensor ``V``, shape ``(3, 3)`` (``:435-448``)."""
+        return self.observables_mod.compute_V_tensor(self)
+
+    def compute_W_tensor(self) -> np.ndarray:
+        """Mixed inertia tensor ``W``, shape ``(3, 3)`` (``:450-463``)."""
+        return self.observables_mod.compute_W_tensor(self)
+
+    def compute_quadrupole_tensor(self) -> np.ndarray:
+        """Isospin-0 electric quadrupole tensor ``Q``, shape ``(3, 3)`` (``:465-478``)."""
+        return self.observables_mod.compute_quadrupole_tensor(self)
+
+    def compute_D_term(self) -> float:
+        """Monopole D-term (``:480-493``)."""
+        return self.observables_mod.compute_D_term(self)
+
     # ------------------------------------------------------------ the hot path
     def step(self, prev_energy: float) -> tuple[float, float]:
         """One arrested-Newton-flow RK4 step (``core/simulation.py:287-308``).

@@ -127,6 +127,20 @@ hipError_t launch_init_field_f32(float* F, float* V, const InitArgs& a, hipStrea
 hipError_t launch_set_component_f64(double* Fc, const double* dens, long long N, double scale, hipStream_t st);
 hipError_t launch_set_component_f32(float* Fc, const double* dens, long long N, double scale, hipStream_t st);
 
+// Post-convergence observables (skx_observables.hip): per-block FP64 partial
+// sums, `nv` values per block, blocks from observable_blocks(N).
+struct ObsArgs {
+  int kind, nv;
+  int X, Y, Z, halo;
+  double lsx, lsy, lsz;
+  double com[3];
+  PhysParams P;
+};
+constexpr int kObsMaxBlocks = 2048;
+int observable_blocks(long long N);
+hipError_t launch_observable_f64(const double* F, double* partial, const ObsArgs& a, int blocks, hipStream_t st);
+hipError_t launch_observable_f32(const float* F, double* partial, const ObsArgs& a, int blocks, hipStream_t st);
+
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);  // zero fill kernel (graph safe)
 hipError_t launch_apply_arrest_f64(double* V, long long n, const DevStatus* st, hipStream_t s);
 hipError_t launch_apply_arrest_f32(float* V, long long n, const DevStatus* st, hipStream_t s);

@@ -143,6 +143,7 @@ struct skx_engine {
   double* partial = nullptr;
   double* eg = nullptr;     // lazily allocated probe / density buffer (double, nf*N)
   double* dens2 = nullptr;  // lazily allocated second density buffer (double, N)
+  double* obs = nullptr;    // lazily allocated observable partials (double, kObsMaxBlocks * 9)
   StepRecord* hist = nullptr;
   int hist_cap = 0;
   // captured chunks of `graph_steps` steps, one per starting parity
@@ -494,6 +495,7 @@ void release(skx_engine* e) {
   if (e->partial) (void)hipFree(e->partial);
   if (e->eg) (void)hipFree(e->eg);
   if (e->dens2) (void)hipFree(e->dens2);
+  if (e->obs) (void)hipFree(e->obs);
   if (e->hist) (void)hipFree(e->hist);
   if (e->stream && e->owns_stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -740,6 +742,57 @@ int skx_density(skx_handle h, int32_t kind, double* host, size_t n) {
 int skx_init_field(skx_handle h, const skx_init_desc* desc) {
   CHECK_HANDLE(h);
   return init_field(h, desc);
+}
+
+int skx_observable(skx_handle h, int32_t kind, const double* com, double* out, int32_t n) {
+  CHECK_HANDLE(h);
+  static const int kCount[] = {4, 1, 9, 9, 9, 9, 1};
+  if (kind < SKX_OBS_COM || kind > SKX_OBS_D) return fail(SKX_EINVAL, "unknown observable kind");
+  if (kind >= SKX_OBS_QUADRUPOLE && h->L.theory != SKX_THEORY_PION)
+    return fail(SKX_EINVAL, "quadrupole, U, V, W and D exist for the nuclear theory only");
+  const int nv = kCount[kind];
+  if (!out || n < nv) return fail(SKX_EINVAL, "output holds fewer values than the observable has");
+  const bool needs_com = kind == SKX_OBS_RMS || (kind >= SKX_OBS_QUADRUPOLE && kind != SKX_OBS_U);
+  if (needs_com && !com) return fail(SKX_EINVAL, "null centre of mass");
+  if (!h->obs) {
+    if (hipMalloc(&h->obs, (size_t)kObsMaxBlocks * 9 * sizeof(double)) != hipSuccess) {
+      h->obs = nullptr;
+      return fail(SKX_ENOMEM, "observable partials allocation failed");
+    }
+  }
+  const Lattice& L = h->L;
+  ObsArgs a{};
+  a.kind = kind;
+  a.nv = nv;
+  a.X = L.X;
+  a.Y = L.Y;
+  a.Z = L.Z;
+  a.halo = L.halo;
+  a.lsx = L.lsx;
+  a.lsy = L.lsy;
+  a.lsz = L.lsz;
+  for (int i = 0; i < 3; ++i) a.com[i] = needs_com ? com[i] : 0.0;
+  a.P = L.P;
+  const int blocks = observable_blocks(L.N);
+  hipError_t r = L.prec == SKX_FP64
+                     ? launch_observable_f64(static_cast<const double*>(h->F[h->cur]), h->obs, a, blocks, h->stream)
+                     : launch_observable_f32(static_cast<const float*>(h->F[h->cur]), h->obs, a, blocks, h->stream);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("observable pass: ") + hipGetErrorString(r));
+  std::vector<double> part((size_t)blocks * nv);
+  HIP_TRY(hipMemcpyAsync(part.data(), h->obs, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  // block partials folded in block order with Neumaier compensation: deterministic
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+      const double v = part[(size_t)b * nv + q];
+      const double t = s + v;
+      c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    out[q] = s + c;
+  }
+  return SKX_OK;
 }
 
 int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n) {

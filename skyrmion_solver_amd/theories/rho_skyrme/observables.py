@@ -1,6 +1,9 @@
-"""Host observables of the Rho-Skyrme model (energy and baryon number reduced on the device in FP64)."""
+"""Host observables of the Rho-Skyrme model: energy and baryon number, plus the centre of mass and
+RMS radius (``theories/rho_skyrme/observables.py:149-238``), all reduced on the device in FP64."""
 
 from __future__ import annotations
+
+from skyrmion_solver_amd.theories._observables import compute_center_of_mass, compute_rms_radius  # noqa: F401
 
 
 def compute_energy(sim) -> float:
