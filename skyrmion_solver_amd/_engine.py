@@ -43,7 +43,7 @@ EXPORTED = (
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
     "skx_batch_remove", "skx_diag_stream_copy", "skx_device_info", "skx_last_error", "skx_version",
-    "skx_init_field", "skx_batch_init", "skx_observable",
+    "skx_init_field", "skx_batch_init", "skx_observable", "skx_write_dat",
 )
 
 
@@ -133,6 +133,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_baryon": ([H, dp], C.c_int),
         "skx_density": ([H, C.c_int32, dp, C.c_size_t], C.c_int),
         "skx_observable": ([H, C.c_int32, dp, dp, C.c_int32], C.c_int),
+        "skx_write_dat": ([C.c_char_p, dp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
+                          C.c_int),
         "skx_eval_gradient": ([H, dp, dp, dp, C.c_size_t], C.c_int),
         "skx_history_attach": ([H, C.c_int32], C.c_int),
         "skx_history_read": ([H, C.POINTER(StepRecord), C.c_int32, C.POINTER(C.c_int32)], C.c_int),

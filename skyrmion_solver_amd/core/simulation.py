@@ -2,8 +2,9 @@
 
 Mirrors ``core/simulation.py`` of the reference (``Simulation.__init__``
 ``:46-122``, ``initialize`` ``:124-173``, ``set_params`` ``:175-197``,
-``observables`` ``:199-217``, the density helpers ``:219-285`` and
-``step`` ``:287-308``) with the same method names, arguments and return
+``observables`` ``:199-217``, the density helpers ``:219-285``, ``step``
+``:287-308``, ``save_output`` ``:310-370`` and the post-convergence
+observables ``:390-493``) with the same method names, arguments and return
 values.  The per-step work runs in the engine's HIP kernels through the C ABI
 (``include/skx.h``); there is no CPU path.
 
@@ -207,14 +208,63 @@ class Simulation:
         """
         return self.engine.anf_run(max_steps, tol, check_every).as_dict()
 
-    def save_output(self, output_dir: str | None = None, precision: int = 32):
-        """Checkpoint the field and velocity as ``.npz`` (the reference's text bundle writer is not rebuilt)."""
+    def save_output(self, output_dir: str | None = None, precision: int = 32, binary: bool = False):
+        """Write the theory's output bundle (``core/simulation.py:310-370``).
+
+        The field, the coordinate grid and the per-site energy and baryon
+        densities (device FP64 passes) go to ``theory.io.output_data_bundle``
+        in the reference's .dat text format and file names; ``output_dir``
+        defaults to ``results/`` beside the theory package, as in the
+        reference.  ``binary=True`` writes ``.npy`` arrays instead of the grid
+        text files (the fast path for large lattices).
+        """
+        import inspect
         from pathlib import Path
 
-        out = Path(output_dir or ".")
-        out.mkdir(parents=True, exist_ok=True)
-        np.savez(out / "state.npz", Field=self.engine.download(_engine.FIELD),
-                 Velocity=self.engine.download(_engine.VELOCITY), p_i=self.p_i_h, p_f=self.p_f_h)
+        from skyrmion_solver_amd.core.io import lattice_grid
+
+        if output_dir is None:
+            output_dir = str(Path(self.theory.io.__file__).resolve().parent / "results")
+        Path(output_dir).mkdir(parents=True, exist_ok=True)
+        X, Y, Z = (int(v) for v in self.p_i_h[:3])
+        kwargs = dict(output_dir=output_dir,
+                      h_Field=self.engine.download(_engine.FIELD),
+                      h_EnergyDensity=self.engine.density(_engine.DENSITY_ENERGY),
+                      h_BaryonDensity=self.engine.density(_engine.DENSITY_BARYON),
+                      h_grid=lattice_grid(X, Y, Z, *(float(v) for v in self.p_f_h[3:6])),
+                      xlen=X, ylen=Y, zlen=Z,
+                      number_coordinates=int(self.p_i_h[4]),
+                      number_total_fields=int(self.p_i_h[5]),
+                      precision=precision,
+                      binary=binary)
+        accepted = set(inspect.signature(self.theory.io.output_data_bundle).parameters)
+        self.theory.io.output_data_bundle(**{k: v for k, v in kwargs.items() if k in accepted})
+
+    def load_output(self, output_dir: str, field_dump_name: str = "Field.dat"):
+        """Restore the field from a bundle written by ``save_output`` and zero the velocity.
+
+        The reference writes no loader (and no velocity), so a restart
+        continues from the field with V = 0.
+        """
+        from skyrmion_solver_amd.core.io import read_field
+
+        X, Y, Z = (int(v) for v in self.p_i_h[:3])
+        F = read_field(output_dir, X, Y, Z, int(self.p_i_h[5]), field_dump_name)
+        self.engine.upload(_engine.FIELD, F)
+        self.engine.upload(_engine.VELOCITY, np.zeros_like(F))
+
+    def save_checkpoint(self, path: str):
+        """Field and velocity, exactly, as ``.npz`` (with ``p_i`` / ``p_f``) for a bit-exact resume."""
+        np.savez(path, Field=self.engine.download(_engine.FIELD), Velocity=self.engine.download(_engine.VELOCITY),
+                 p_i=self.p_i_h, p_f=self.p_f_h)
+
+    def load_checkpoint(self, path: str):
+        """Restore a ``save_checkpoint`` file written for the same lattice."""
+        with np.load(path, allow_pickle=False) as d:
+            if not np.array_equal(d["p_i"], self.p_i_h):
+                raise ValueError("checkpoint was written for different integer parameters (lattice / theory)")
+            self.engine.upload(_engine.FIELD, d["Field"])
+            self.engine.upload(_engine.VELOCITY, d["Velocity"])
 
     def print_instructions(self):
         if hasattr(self.theory, "print_instructions"):

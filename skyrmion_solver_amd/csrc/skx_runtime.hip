@@ -600,6 +600,11 @@ int create_engine(const skx_desc* desc, hipStream_t shared, DevStatus* ext_st, D
 
 }  // namespace
 
+// Lets the host-only sources (skx_io.cpp) report through skx_last_error.
+namespace skx {
+int record_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace skx
+
 extern "C" {
 
 int skx_version(void) { return SKX_VERSION; }

@@ -4,7 +4,7 @@ from __future__ import annotations
 
 from skyrmion_solver_amd.theories.registry import get_theory_spec
 
-from . import initial_config, kernels, observables, params  # noqa: F401
+from . import initial_config, io, kernels, observables, params  # noqa: F401
 
 THEORY_SPEC = get_theory_spec("Rho-Skyrme model")
 threads3d = (8, 8, 4)
