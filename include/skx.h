@@ -42,7 +42,9 @@ enum skx_theory {
   SKX_THEORY_PION = 0,    /* theories/nuclear_skyrme  ("Nuclear Skyrme model") */
   SKX_THEORY_OMEGA = 1,   /* theories/omega_skyrme    ("Omega-Skyrme model")   */
   SKX_THEORY_COULOMB = 2, /* theories/coulomb_skyrme  ("Coulomb-Skyrme model") */
-  SKX_THEORY_RHO = 3      /* theories/rho_skyrme      ("Rho-Skyrme model")     */
+  SKX_THEORY_RHO = 3,     /* theories/rho_skyrme      ("Rho-Skyrme model")     */
+  SKX_THEORY_BERGER = 4,  /* theories/berger_skyrme   ("Berger Skyrme model"): p_f[12] = alpha */
+  SKX_THEORY_LIGHTLY_BOUND = 5 /* theories/lightly_bound_skyrme ("Lightly bound Skyrme model"): p_f[9] = alpha */
 };
 
 enum skx_precision {

@@ -1,4 +1,4 @@
-"""Per-site physics of the four theories (oracle; test infrastructure only).
+"""Per-site physics of the six theories (oracle; test infrastructure only).
 
 Every function works on whole grids: ``phi`` is ``(nf, X, Y, Z)``, ``d1`` is
 ``(3, nf, X, Y, Z)`` and ``d2`` is ``(3, 3, nf, X, Y, Z)`` (see
@@ -16,6 +16,7 @@ import math
 import numpy as np
 
 PION, OMEGA, COULOMB, RHO = "pion", "omega", "coulomb", "rho"
+BERGER, LIGHTLY = "berger", "lightly"  # Berger-Skyrme, lightly bound Skyrme
 
 
 # ---------------------------------------------------------------- helpers
@@ -81,8 +82,8 @@ def n_sigma_fields(theory: str, p_i) -> int:
     omega / Coulomb: ``p_i[11]`` (``theories/omega_skyrme/kernels.py:179,221``);
     rho: hard-coded 4 (``theories/rho_skyrme/kernels.py:243-288``).
     """
-    if theory == PION:
-        return int(p_i[5])
+    if theory in (PION, BERGER, LIGHTLY):
+        return int(p_i[5])  # berger / lightly bound: kernels.py compute_norm_skyrme_field, same as nuclear
     if theory in (OMEGA, COULOMB):
         return int(p_i[11])
     return 4
@@ -368,7 +369,104 @@ def gradient_rho(phi, d1, d2, p_i, p_f):
     return EG, -dt * EG
 
 
-GRADIENTS = {PION: gradient_pion, OMEGA: gradient_omega, COULOMB: gradient_coulomb, RHO: gradient_rho}
+def _omega4(a, b):
+    """``_omega_contract`` (``theories/berger_skyrme/kernels.py:81-111``): a0 b3 + a1 b2 - a2 b1 - a3 b0."""
+    return a[0] * b[3] + a[1] * b[2] - a[2] * b[1] - a[3] * b[0]
+
+
+def _omega_vec(v):
+    """The vector Omega v with a.(Omega v) = omega(a, v): (v3, v2, -v1, -v0) (``berger_skyrme/kernels.py:659-677``)."""
+    return [v[3], v[2], -v[1], -v[0]]
+
+
+def _quartic_terms(F, S, nf, like):
+    """The i != j loop of the standard quartic variation, per component, without its coupling factor."""
+    norm = [sum(F[i, b] * F[i, b] for b in range(nf)) for i in range(3)]
+    out = [np.zeros_like(like) for _ in range(nf)]
+    for i in range(3):
+        for j in range(3):
+            if i == j:
+                continue
+            dnormj_di = np.zeros_like(like)
+            ddotij_di = np.zeros_like(like)
+            dotij = np.zeros_like(like)
+            for b in range(nf):
+                dnormj_di = dnormj_di + 2.0 * F[j, b] * S[i, j, b]
+                ddotij_di = ddotij_di + (S[i, i, b] * F[j, b] + F[i, b] * S[i, j, b])
+                dotij = dotij + F[i, b] * F[j, b]
+            for a in range(nf):
+                out[a] = out[a] + (dnormj_di * F[i, a] + norm[j] * S[i, i, a] - ddotij_di * F[j, a]
+                                   - dotij * S[i, j, a])
+    return out
+
+
+def gradient_berger(phi, d1, d2, p_i, p_f):
+    """``do_gradient_step_point`` (``theories/berger_skyrme/kernels.py:554-737``).
+
+    Nuclear gradient plus the squashing terms (beta = alpha^2 - 1, alpha =
+    ``p_f[12]``): ``-beta [2 sum_i u_i Omega F_i + (sum_i omega(S_ii, phi)) Omega phi]``
+    with ``u_i = omega(F_i, phi)`` (``:642-682``) and
+    ``-kappa^2 beta sum_{i<j} [d_i W_ij Omega F_j - d_j W_ij Omega F_i]`` with
+    ``W_ij = omega(F_i, F_j)`` (``:705-724``); then ``- mpi^2`` on sigma and the
+    tangent projection.
+    """
+    dt = p_f[7]
+    mpi2 = p_f[8] * p_f[8]
+    kappa2 = p_f[9] * p_f[9]
+    alpha = p_f[12]
+    beta = alpha * alpha - 1.0
+    F, S = d1, d2
+    g = [-(S[0, 0, a] + S[1, 1, a] + S[2, 2, a]) for a in range(4)]
+    u = [_omega4(F[i], phi) for i in range(3)]
+    div_u = _omega4(S[0, 0], phi) + _omega4(S[1, 1], phi) + _omega4(S[2, 2], phi)
+    op = _omega_vec(phi)
+    of = [_omega_vec(F[i]) for i in range(3)]
+    for a in range(4):
+        g[a] = g[a] - beta * (2.0 * u[0] * of[0][a] + 2.0 * u[1] * of[1][a] + 2.0 * u[2] * of[2][a]
+                              + div_u * op[a])
+    q = _quartic_terms(F, S, 4, phi[0])
+    for a in range(4):
+        g[a] = g[a] - kappa2 * q[a]
+    pairs = [(0, 1), (0, 2), (1, 2)]
+    for a in range(4):
+        acc = np.zeros_like(phi[0])
+        for i, j in pairs:
+            dWi = _omega4(S[i, i], F[j]) + _omega4(F[i], S[i, j])  # d_i W_ij
+            dWj = _omega4(S[i, j], F[j]) + _omega4(F[i], S[j, j])  # d_j W_ij
+            acc = acc + (dWi * of[j][a] - dWj * of[i][a])
+        g[a] = g[a] - kappa2 * beta * acc
+    g[0] = g[0] - mpi2
+    EG = np.array(g)
+    project(EG, phi, 4)
+    return EG, -dt * EG
+
+
+def gradient_lightly(phi, d1, d2, p_i, p_f):
+    """``do_gradient_step_point`` (``theories/lightly_bound_skyrme/kernels.py:567-683``).
+
+    ``-2 (1 - alpha) Laplacian phi`` minus the quartic variation with unit
+    coupling, then ``- 2 (1 - alpha) mpi^2 - 4 alpha (1 - sigma)^3`` on sigma
+    (alpha = ``p_f[9]``) and the tangent projection.
+    """
+    nf = int(p_i[5])
+    dt = p_f[7]
+    mpi2 = p_f[8] * p_f[8]
+    alpha = p_f[9]
+    F, S = d1, d2
+    g = [-2.0 * (1.0 - alpha) * (S[0, 0, a] + S[1, 1, a] + S[2, 2, a]) for a in range(nf)]
+    q = _quartic_terms(F, S, nf, phi[0])
+    for a in range(nf):
+        g[a] = g[a] - q[a]
+    s0 = phi[0]
+    g[0] = g[0] - (1.0 - alpha) * 2.0 * mpi2
+    g[0] = g[0] - 4.0 * alpha * (1.0 - s0) * (1.0 - s0) * (1.0 - s0)
+    EG = np.array(g)
+    project(EG, phi, nf)
+    return EG, -dt * EG
+
+
+GRADIENTS = {PION: gradient_pion, OMEGA: gradient_omega, COULOMB: gradient_coulomb, RHO: gradient_rho,
+             BERGER: gradient_berger, LIGHTLY: gradient_lightly}
 
 
 # ---------------------------------------------------------------- energy densities
@@ -464,14 +562,59 @@ def energy_rho(phi, d1, p_i, p_f):
     return energy * vol
 
 
-ENERGIES = {PION: energy_pion, OMEGA: energy_omega, COULOMB: energy_coulomb, RHO: energy_rho}
+def energy_berger(phi, d1, p_i, p_f):
+    """``compute_energy_point`` (``theories/berger_skyrme/kernels.py:251-337``), times vol.
+
+    Nuclear energy plus ``beta/2 sum_i u_i^2 + kappa^2 beta/2 sum_{i<j} W_ij^2``.
+    """
+    vol = p_f[6]
+    mpi2 = p_f[8] * p_f[8]
+    kappa2 = p_f[9] * p_f[9]
+    alpha = p_f[12]
+    beta = alpha * alpha - 1.0
+    f = [[d1[i, a] for a in range(4)] for i in range(3)]
+    nx, ny, nz = dot4(f[0], f[0]), dot4(f[1], f[1]), dot4(f[2], f[2])
+    dxy, dxz, dyz = dot4(f[0], f[1]), dot4(f[0], f[2]), dot4(f[1], f[2])
+    ux, uy, uz = (_omega4(f[i], phi) for i in range(3))
+    wxy, wxz, wyz = _omega4(f[0], f[1]), _omega4(f[0], f[2]), _omega4(f[1], f[2])
+    energy = 0.5 * (nx + ny + nz)
+    energy = energy + 0.5 * beta * (ux * ux + uy * uy + uz * uz)
+    energy = energy + 0.5 * kappa2 * ((nx * ny - dxy * dxy) + (nx * nz - dxz * dxz) + (ny * nz - dyz * dyz))
+    energy = energy + 0.5 * kappa2 * beta * (wxy * wxy + wxz * wxz + wyz * wyz)
+    energy = energy + mpi2 * (1.0 - phi[0])
+    return energy * vol
+
+
+def energy_lightly(phi, d1, p_i, p_f):
+    """``compute_energy_point`` (``theories/lightly_bound_skyrme/kernels.py:275-345``), times vol.
+
+    ``(1 - alpha)|d phi|^2 + quartic/2 + 2 (1 - alpha) mpi^2 (1 - sigma) + alpha (1 - sigma)^4``.
+    """
+    vol = p_f[6]
+    mpi2 = p_f[8] * p_f[8]
+    alpha = p_f[9]
+    f = [[d1[i, a] for a in range(4)] for i in range(3)]
+    nx, ny, nz = dot4(f[0], f[0]), dot4(f[1], f[1]), dot4(f[2], f[2])
+    dxy, dxz, dyz = dot4(f[0], f[1]), dot4(f[0], f[2]), dot4(f[1], f[2])
+    om = 1.0 - phi[0]
+    energy = (1.0 - alpha) * (nx + ny + nz)
+    energy = energy + 0.5 * ((nx * ny - dxy * dxy) + (nx * nz - dxz * dxz) + (ny * nz - dyz * dyz))
+    energy = energy + (1.0 - alpha) * 2.0 * mpi2 * om
+    energy = energy + alpha * om * om * om * om
+    return energy * vol
+
+
+ENERGIES = {PION: energy_pion, OMEGA: energy_omega, COULOMB: energy_coulomb, RHO: energy_rho,
+            BERGER: energy_berger, LIGHTLY: energy_lightly}
 
 
 def baryon_density(theory, phi, d1, p_i, p_f):
     """Per-site baryon density times the cell volume.
 
-    nuclear / rho: ``-det[phi, F_x, F_y, F_z] * (vol / (2 pi^2))``
-    (``theories/nuclear_skyrme/kernels.py:441-447``); omega / Coulomb:
+    nuclear / rho / Berger / lightly bound: ``-det[phi, F_x, F_y, F_z] * (vol / (2 pi^2))``
+    (``theories/nuclear_skyrme/kernels.py:441-447``,
+    ``theories/berger_skyrme/kernels.py:374-433``,
+    ``theories/lightly_bound_skyrme/kernels.py:383-446``); omega / Coulomb:
     ``vol * CS`` (``theories/omega_skyrme/kernels.py:374-375``).
     """
     vol = p_f[6]

@@ -16,7 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libskx.so"
 SOURCES = ["skx_kernels.hip", "skx_runtime.hip", "skx_diag.hip", "skx_init.hip", "skx_observables.hip", "skx_io.cpp"] + [
-    f"skx_stage_{th}_{p}.hip" for th in ("pion", "omega", "coulomb", "rho") for p in ("f32", "f64")]
+    f"skx_stage_{th}_{p}.hip" for th in ("pion", "omega", "coulomb", "rho", "berger", "lightly") for p in ("f32", "f64")]
 ARCH = os.environ.get("SKX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -26,6 +26,7 @@ ERRORS = {
 }
 
 THEORY_PION, THEORY_OMEGA, THEORY_COULOMB, THEORY_RHO = 0, 1, 2, 3
+THEORY_BERGER, THEORY_LIGHTLY_BOUND = 4, 5
 FP64, FP32 = 0, 1
 FIELD, VELOCITY = 0, 1
 DENSITY_ENERGY, DENSITY_BARYON, DENSITY_POTENTIAL, DENSITY_RHO = 0, 1, 2, 3

@@ -56,6 +56,8 @@ hipError_t launch_stage_f64(int theory, int stage, const StageArgs<double>& a, c
     case OMEGA: return stage_launch<double, OMEGA>(stage, a, s, st);
     case COULOMB: return stage_launch<double, COULOMB>(stage, a, s, st);
     case RHO: return stage_launch<double, RHO>(stage, a, s, st);
+    case BERGER: return stage_launch<double, BERGER>(stage, a, s, st);
+    case LIGHTLY: return stage_launch<double, LIGHTLY>(stage, a, s, st);
   }
   return hipErrorInvalidValue;
 }
@@ -65,6 +67,8 @@ hipError_t launch_stage_f32(int theory, int stage, const StageArgs<float>& a, co
     case OMEGA: return stage_launch<float, OMEGA>(stage, a, s, st);
     case COULOMB: return stage_launch<float, COULOMB>(stage, a, s, st);
     case RHO: return stage_launch<float, RHO>(stage, a, s, st);
+    case BERGER: return stage_launch<float, BERGER>(stage, a, s, st);
+    case LIGHTLY: return stage_launch<float, LIGHTLY>(stage, a, s, st);
   }
   return hipErrorInvalidValue;
 }
@@ -76,6 +80,8 @@ hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, c
     case OMEGA: return stage_launch_batch<double, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
     case COULOMB: return stage_launch_batch<double, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
     case RHO: return stage_launch_batch<double, RHO>(stage, s, slots, blk_inst, parity, grid, st);
+    case BERGER: return stage_launch_batch<double, BERGER>(stage, s, slots, blk_inst, parity, grid, st);
+    case LIGHTLY: return stage_launch_batch<double, LIGHTLY>(stage, s, slots, blk_inst, parity, grid, st);
   }
   return hipErrorInvalidValue;
 }
@@ -86,6 +92,8 @@ hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, c
     case OMEGA: return stage_launch_batch<float, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
     case COULOMB: return stage_launch_batch<float, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
     case RHO: return stage_launch_batch<float, RHO>(stage, s, slots, blk_inst, parity, grid, st);
+    case BERGER: return stage_launch_batch<float, BERGER>(stage, s, slots, blk_inst, parity, grid, st);
+    case LIGHTLY: return stage_launch_batch<float, LIGHTLY>(stage, s, slots, blk_inst, parity, grid, st);
   }
   return hipErrorInvalidValue;
 }
@@ -96,6 +104,8 @@ hipError_t stage_blocks_per_cu(int prec_fp32, int theory, int stage, const Launc
     case OMEGA: return prec_fp32 ? stage_occupancy<float, OMEGA>(stage, s, blocks) : stage_occupancy<double, OMEGA>(stage, s, blocks);
     case COULOMB: return prec_fp32 ? stage_occupancy<float, COULOMB>(stage, s, blocks) : stage_occupancy<double, COULOMB>(stage, s, blocks);
     case RHO: return prec_fp32 ? stage_occupancy<float, RHO>(stage, s, blocks) : stage_occupancy<double, RHO>(stage, s, blocks);
+    case BERGER: return prec_fp32 ? stage_occupancy<float, BERGER>(stage, s, blocks) : stage_occupancy<double, BERGER>(stage, s, blocks);
+    case LIGHTLY: return prec_fp32 ? stage_occupancy<float, LIGHTLY>(stage, s, blocks) : stage_occupancy<double, LIGHTLY>(stage, s, blocks);
   }
   return hipErrorInvalidValue;
 }

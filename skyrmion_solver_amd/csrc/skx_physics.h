@@ -20,7 +20,7 @@ namespace skx {
 
 constexpr double kPi = 3.14159265358979323846;
 
-enum Theory : int { PION = 0, OMEGA = 1, COULOMB = 2, RHO = 3 };
+enum Theory : int { PION = 0, OMEGA = 1, COULOMB = 2, RHO = 3, BERGER = 4, LIGHTLY = 5 };
 
 // Index of S for the pair (i, j) in the packed 6-entry symmetric layout.
 SKX_HD constexpr int sidx(int i, int j) {
@@ -72,7 +72,7 @@ SKX_HD T det_phi_F(const T* phi, const T (*F)[4]) {
 // with n_i = F_i.F_i, n = sum n_i, d_ij = F_i.F_j and
 //   C_k = sum_{i != k} (F_i.S_ik - S_ii.F_k).
 template <typename T, int LD>
-SKX_HD void skyrme_quartic_gradient(const T (*F)[LD], const T (*S)[LD], T k2, T g[4]) {
+SKX_HD void skyrme_quartic_gradient_q(const T (*F)[LD], const T (*S)[LD], T q, T k2, T g[4]) {
   T n[3], d[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) n[i] = F[i][0] * F[i][0] + F[i][1] * F[i][1] + F[i][2] * F[i][2] + F[i][3] * F[i][3];
@@ -95,9 +95,9 @@ SKX_HD void skyrme_quartic_gradient(const T (*F)[LD], const T (*S)[LD], T k2, T 
     }
     C[k] = c;
   }
-  const T w0 = T(1) + k2 * (ntot - n[0]);
-  const T w1 = T(1) + k2 * (ntot - n[1]);
-  const T w2 = T(1) + k2 * (ntot - n[2]);
+  const T w0 = q + k2 * (ntot - n[0]);
+  const T w1 = q + k2 * (ntot - n[1]);
+  const T w2 = q + k2 * (ntot - n[2]);
   const T e01 = T(2) * k2 * d[0], e02 = T(2) * k2 * d[1], e12 = T(2) * k2 * d[2];
   const T c0 = k2 * C[0], c1 = k2 * C[1], c2 = k2 * C[2];
 #pragma unroll
@@ -105,6 +105,12 @@ SKX_HD void skyrme_quartic_gradient(const T (*F)[LD], const T (*S)[LD], T k2, T 
     g[a] = -(w0 * S[0][a] + w1 * S[1][a] + w2 * S[2][a]) - (c0 * F[0][a] + c1 * F[1][a] + c2 * F[2][a]) +
            (e01 * S[3][a] + e02 * S[4][a] + e12 * S[5][a]);
   }
+}
+
+// The same with the sigma-model Laplacian at unit weight (nuclear, Coulomb, Berger).
+template <typename T, int LD>
+SKX_HD void skyrme_quartic_gradient(const T (*F)[LD], const T (*S)[LD], T k2, T g[4]) {
+  skyrme_quartic_gradient_q<T, LD>(F, S, T(1), k2, g);
 }
 
 // g <- g - (g.phi) phi over the first NSK components
@@ -156,6 +162,56 @@ template <typename T>
 SKX_HD void gradient_pion(const T* phi, const T (*F)[4], const T (*S)[4], const PhysParams& P, T g[4]) {
   skyrme_quartic_gradient<T, 4>(F, S, T(P.k2), g);
   g[0] -= T(P.m2);
+  project_tangent<T, 4>(g, phi);
+}
+
+// ---------------------------------------------------------------- Berger-Skyrme
+// omega(a, b) = a0 b3 + a1 b2 - a2 b1 - a3 b0 (_omega_contract,
+// theories/berger_skyrme/kernels.py:81-111); Omega v = (v3, v2, -v1, -v0) is
+// the vector with a . (Omega v) = omega(a, v) (:659-677).
+template <typename T>
+SKX_HD T omega4(const T* a, const T* b) {
+  return a[0] * b[3] + a[1] * b[2] - a[2] * b[1] - a[3] * b[0];
+}
+
+// theories/berger_skyrme/kernels.py:554-737: the nuclear gradient plus, with
+// beta = alpha^2 - 1 (P.coup) and u_i = omega(F_i, phi), W_ij = omega(F_i, F_j),
+//   - beta [2 sum_i u_i Omega F_i + (sum_i omega(S_ii, phi)) Omega phi]         (:642-682)
+//   - kappa^2 beta sum_{i<j} [d_i W_ij Omega F_j - d_j W_ij Omega F_i]           (:705-724)
+// with d_k W_ij = omega(S_ik, F_j) + omega(F_i, S_jk), gathered here per Omega F_i.
+template <typename T>
+SKX_HD void gradient_berger(const T* phi, const T (*F)[4], const T (*S)[4], const PhysParams& P, T g[4]) {
+  skyrme_quartic_gradient<T, 4>(F, S, T(P.k2), g);
+  const T beta = T(P.coup), kb = T(P.k2) * beta;
+  const T du = omega4<T>(S[0], phi) + omega4<T>(S[1], phi) + omega4<T>(S[2], phi);
+  const T dWxy_x = omega4<T>(S[0], F[1]) + omega4<T>(F[0], S[3]);
+  const T dWxy_y = omega4<T>(S[3], F[1]) + omega4<T>(F[0], S[1]);
+  const T dWxz_x = omega4<T>(S[0], F[2]) + omega4<T>(F[0], S[4]);
+  const T dWxz_z = omega4<T>(S[4], F[2]) + omega4<T>(F[0], S[2]);
+  const T dWyz_y = omega4<T>(S[1], F[2]) + omega4<T>(F[1], S[5]);
+  const T dWyz_z = omega4<T>(S[5], F[2]) + omega4<T>(F[1], S[2]);
+  const T cx = T(2) * beta * omega4<T>(F[0], phi) - kb * (dWxy_y + dWxz_z);
+  const T cy = T(2) * beta * omega4<T>(F[1], phi) + kb * (dWxy_x - dWyz_z);
+  const T cz = T(2) * beta * omega4<T>(F[2], phi) + kb * (dWxz_x + dWyz_y);
+  const T cp = beta * du;
+  // Omega v = (v3, v2, -v1, -v0)
+  g[0] -= cx * F[0][3] + cy * F[1][3] + cz * F[2][3] + cp * phi[3];
+  g[1] -= cx * F[0][2] + cy * F[1][2] + cz * F[2][2] + cp * phi[2];
+  g[2] += cx * F[0][1] + cy * F[1][1] + cz * F[2][1] + cp * phi[1];
+  g[3] += cx * F[0][0] + cy * F[1][0] + cz * F[2][0] + cp * phi[0];
+  g[0] -= T(P.m2);
+  project_tangent<T, 4>(g, phi);
+}
+
+// theories/lightly_bound_skyrme/kernels.py:567-683: sigma-model term at weight
+// 2 (1 - alpha), quartic term at unit coupling, then
+// - 2 (1 - alpha) mpi^2 - 4 alpha (1 - sigma)^3 on sigma; alpha = P.coup.
+template <typename T>
+SKX_HD void gradient_lightly(const T* phi, const T (*F)[4], const T (*S)[4], const PhysParams& P, T g[4]) {
+  const T al = T(P.coup), one_al = T(1) - al;
+  skyrme_quartic_gradient_q<T, 4>(F, S, T(2) * one_al, T(1), g);
+  const T om = T(1) - phi[0];
+  g[0] -= T(2) * one_al * T(P.m2) + T(4) * al * (om * om * om);
   project_tangent<T, 4>(g, phi);
 }
 
@@ -373,6 +429,53 @@ SKX_HD double energy_pion(const T* phi, const T (*F)[LD], const PhysParams& P) {
   double e = 0.5 * (nx + ny + nz);
   e += 0.5 * P.k2 * ((nx * ny - dxy * dxy) + (nx * nz - dxz * dxz) + (ny * nz - dyz * dyz));
   e += P.m2 * (1.0 - double(phi[0]));
+  return e * P.vol;
+}
+
+// theories/berger_skyrme/kernels.py:251-337: the nuclear density plus
+// beta/2 sum_i u_i^2 + kappa^2 beta/2 sum_{i<j} W_ij^2.
+template <typename T, int LD>
+SKX_HD double energy_berger(const T* phi, const T (*F)[LD], const PhysParams& P) {
+  double f[3][4], s[4];
+  for (int a = 0; a < 4; ++a) {
+    s[a] = double(phi[a]);
+    for (int i = 0; i < 3; ++i) f[i][a] = double(F[i][a]);
+  }
+  const double nx = f[0][0] * f[0][0] + f[0][1] * f[0][1] + f[0][2] * f[0][2] + f[0][3] * f[0][3];
+  const double ny = f[1][0] * f[1][0] + f[1][1] * f[1][1] + f[1][2] * f[1][2] + f[1][3] * f[1][3];
+  const double nz = f[2][0] * f[2][0] + f[2][1] * f[2][1] + f[2][2] * f[2][2] + f[2][3] * f[2][3];
+  const double dxy = f[0][0] * f[1][0] + f[0][1] * f[1][1] + f[0][2] * f[1][2] + f[0][3] * f[1][3];
+  const double dxz = f[0][0] * f[2][0] + f[0][1] * f[2][1] + f[0][2] * f[2][2] + f[0][3] * f[2][3];
+  const double dyz = f[1][0] * f[2][0] + f[1][1] * f[2][1] + f[1][2] * f[2][2] + f[1][3] * f[2][3];
+  const double ux = omega4<double>(f[0], s), uy = omega4<double>(f[1], s), uz = omega4<double>(f[2], s);
+  const double wxy = omega4<double>(f[0], f[1]), wxz = omega4<double>(f[0], f[2]), wyz = omega4<double>(f[1], f[2]);
+  const double beta = P.coup;
+  double e = 0.5 * (nx + ny + nz);
+  e += 0.5 * beta * (ux * ux + uy * uy + uz * uz);
+  e += 0.5 * P.k2 * ((nx * ny - dxy * dxy) + (nx * nz - dxz * dxz) + (ny * nz - dyz * dyz));
+  e += 0.5 * P.k2 * beta * (wxy * wxy + wxz * wxz + wyz * wyz);
+  e += P.m2 * (1.0 - s[0]);
+  return e * P.vol;
+}
+
+// theories/lightly_bound_skyrme/kernels.py:275-345:
+// (1 - alpha)|d phi|^2 + quartic / 2 + 2 (1 - alpha) mpi^2 (1 - sigma) + alpha (1 - sigma)^4.
+template <typename T, int LD>
+SKX_HD double energy_lightly(const T* phi, const T (*F)[LD], const PhysParams& P) {
+  double f[3][4];
+  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < 4; ++a) f[i][a] = double(F[i][a]);
+  const double nx = f[0][0] * f[0][0] + f[0][1] * f[0][1] + f[0][2] * f[0][2] + f[0][3] * f[0][3];
+  const double ny = f[1][0] * f[1][0] + f[1][1] * f[1][1] + f[1][2] * f[1][2] + f[1][3] * f[1][3];
+  const double nz = f[2][0] * f[2][0] + f[2][1] * f[2][1] + f[2][2] * f[2][2] + f[2][3] * f[2][3];
+  const double dxy = f[0][0] * f[1][0] + f[0][1] * f[1][1] + f[0][2] * f[1][2] + f[0][3] * f[1][3];
+  const double dxz = f[0][0] * f[2][0] + f[0][1] * f[2][1] + f[0][2] * f[2][2] + f[0][3] * f[2][3];
+  const double dyz = f[1][0] * f[2][0] + f[1][1] * f[2][1] + f[1][2] * f[2][2] + f[1][3] * f[2][3];
+  const double al = P.coup, om = 1.0 - double(phi[0]);
+  double e = (1.0 - al) * (nx + ny + nz);
+  e += 0.5 * ((nx * ny - dxy * dxy) + (nx * nz - dxz * dxz) + (ny * nz - dyz * dyz));
+  e += (1.0 - al) * 2.0 * P.m2 * om;
+  e += al * om * om * om * om;
   return e * P.vol;
 }
 

@@ -96,6 +96,19 @@ int parse_desc(const skx_desc* d, Lattice& L) {
       L.P.coup = pf[9] * pf[9];
       L.P.calpha = pf[10];
       break;
+    case SKX_THEORY_BERGER:
+      if (L.nf != 4) return fail(SKX_EINVAL, "Berger theory needs number_total_fields == 4");
+      if (d->n_f < 13) return fail(SKX_EINVAL, "Berger theory needs p_f[8] (mpi), p_f[9] (kappa), p_f[12] (alpha)");
+      L.P.m2 = pf[8] * pf[8];
+      L.P.k2 = pf[9] * pf[9];
+      L.P.coup = pf[12] * pf[12] - 1.0;  // beta (theories/berger_skyrme/kernels.py:602)
+      break;
+    case SKX_THEORY_LIGHTLY_BOUND:
+      if (L.nf != 4) return fail(SKX_EINVAL, "lightly bound theory needs number_total_fields == 4");
+      if (d->n_f < 10) return fail(SKX_EINVAL, "lightly bound theory needs p_f[8] (mpi) and p_f[9] (alpha)");
+      L.P.m2 = pf[8] * pf[8];
+      L.P.coup = pf[9];
+      break;
     default:
       return fail(SKX_EINVAL, "unknown theory id");
   }
@@ -1028,8 +1041,7 @@ extern "C" {
 int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   if (!d || !out) return fail(SKX_EINVAL, "null argument");
   *out = nullptr;
-  if (d->theory != SKX_THEORY_PION && d->theory != SKX_THEORY_OMEGA && d->theory != SKX_THEORY_COULOMB &&
-      d->theory != SKX_THEORY_RHO)
+  if (d->theory < SKX_THEORY_PION || d->theory > SKX_THEORY_LIGHTLY_BOUND)
     return fail(SKX_EINVAL, "unknown theory id");
   if (d->precision != SKX_FP64 && d->precision != SKX_FP32) return fail(SKX_EINVAL, "unknown precision");
   if (d->max_slots < 1 || d->max_blocks < 1) return fail(SKX_EINVAL, "max_slots and max_blocks must be positive");

@@ -1,5 +1,5 @@
 // Stage kernel templates of the arrested-Newton-flow step for gfx950
-// (pion, omega and Coulomb theories).
+// (pion, omega, Coulomb, rho, Berger and lightly bound theories).
 //
 // One kernel template covers the energy pass (stage 0) and the four RK4
 // gradient stages of one step.  Each stage is one pass over the lattice:
@@ -61,6 +61,18 @@ struct Traits<RHO> {
   static constexpr unsigned MIXED = 0xFu;
 };
 
+// Berger-Skyrme and lightly bound: the pion layout, new point functions only.
+template <>
+struct Traits<BERGER> {
+  static constexpr int NF = 4, NST = 4, NSK = 4, NAUX = 0;
+  static constexpr unsigned MIXED = 0xFu;
+};
+template <>
+struct Traits<LIGHTLY> {
+  static constexpr int NF = 4, NST = 4, NSK = 4, NAUX = 0;
+  static constexpr unsigned MIXED = 0xFu;
+};
+
 template <int TH, typename T, int LD>
 __device__ __forceinline__ void point_gradient(const T* phi, const T (*F)[LD], const T (*S)[LD], const T (*aux)[3],
                                                const PhysParams& P, T* g) {
@@ -68,6 +80,8 @@ __device__ __forceinline__ void point_gradient(const T* phi, const T (*F)[LD], c
   if constexpr (TH == OMEGA) gradient_omega<T>(phi, F, S, P, g);
   if constexpr (TH == COULOMB) gradient_coulomb<T>(phi, F, S, P, g);
   if constexpr (TH == RHO) gradient_rho_split<T>(phi, F, S, aux, P, g);
+  if constexpr (TH == BERGER) gradient_berger<T>(phi, F, S, P, g);
+  if constexpr (TH == LIGHTLY) gradient_lightly<T>(phi, F, S, P, g);
 }
 
 template <int TH, typename T, int LD>
@@ -77,6 +91,8 @@ __device__ __forceinline__ double point_energy(const T* phi, const T (*F)[LD], c
   if constexpr (TH == OMEGA) return energy_omega<T, LD>(phi, F, P);
   if constexpr (TH == COULOMB) return energy_coulomb<T, LD>(phi, F, P);
   if constexpr (TH == RHO) return energy_rho_split<T>(phi, F, aux, P);
+  if constexpr (TH == BERGER) return energy_berger<T, LD>(phi, F, P);
+  if constexpr (TH == LIGHTLY) return energy_lightly<T, LD>(phi, F, P);
   return 0.0;
 }
 

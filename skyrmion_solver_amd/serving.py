@@ -32,7 +32,7 @@ import numpy as np
 from skyrmion_solver_amd import _engine
 
 THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
-              "rho": _engine.THEORY_RHO}
+              "rho": _engine.THEORY_RHO, "berger": _engine.THEORY_BERGER, "lightly": _engine.THEORY_LIGHTLY_BOUND}
 
 
 @dataclass

@@ -1,6 +1,6 @@
 """Public theory interface: ``load_theory``, ``list``, ``print_table``.
 
-Mirrors ``theories/__init__.py`` of the reference.  The four theories whose
+Mirrors ``theories/__init__.py`` of the reference.  The six theories whose
 hot path the engine runs are registered here with the reference's names and
 aliases (``theories/*/__init__.py:20-26``).
 """
@@ -25,6 +25,12 @@ _BUILTIN = (
     TheorySpec(name="Rho-Skyrme model", aliases=("Rho-Skyrme", "Rho-skyrmions", "Skyrmions with rho mesons"),
                import_path="skyrmion_solver_amd.theories.rho_skyrme",
                description="Skyrmions coupled to rho mesons.", version="1.0"),
+    TheorySpec(name="Berger Skyrme model", aliases=("Berger Skyrme", "Squashed sphere Skyrme", "Berger skyrmions"),
+               import_path="skyrmion_solver_amd.theories.berger_skyrme",
+               description="Skyrmions in the 3D Berger-Skyrme model, with the standard pion mass potential.", version="1.0"),
+    TheorySpec(name="Lightly bound Skyrme model", aliases=("Lightly bound Skyrme", "Loosely bound Skyrme", "Loosely bound Skyrme model"),
+               import_path="skyrmion_solver_amd.theories.lightly_bound_skyrme",
+               description="Skyrmions in the lightly bound Skyrme model.", version="1.0"),
 )
 
 for _spec in _BUILTIN:

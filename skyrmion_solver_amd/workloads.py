@@ -17,9 +17,9 @@ from skyrmion_solver_amd import initial_data
 from skyrmion_solver_amd.theories import load_theory
 
 COURANT = 0.2
-COURANT_BY_THEORY = {"pion": 0.2, "omega": 0.2, "coulomb": 0.2, "rho": 0.1}
+COURANT_BY_THEORY = {"pion": 0.2, "omega": 0.2, "coulomb": 0.2, "rho": 0.1, "berger": 0.2, "lightly": 0.2}
 THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model", "coulomb": "Coulomb-Skyrme model",
-                "rho": "Rho-Skyrme model"}
+                "rho": "Rho-Skyrme model", "berger": "Berger Skyrme model", "lightly": "Lightly bound Skyrme model"}
 SERVE_MIX = (("pion", 0.60), ("omega", 0.25), ("rho", 0.15))
 
 
@@ -135,6 +135,12 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
     elif theory == "rho":
         lat = _lattice(L, 0.15, 0.1)
         extra = dict(e_skyrme=3.65, f_pi=129.0, mass_pion=138.0, mass_rho=775.0, alpha=float(rng.uniform(0.055, 0.068)))
+    elif theory == "berger":
+        lat = _lattice(L, 0.15, 0.2)
+        extra = dict(mpi=float(rng.uniform(0.5, 1.5)), kappa=1.0, alpha=float(rng.uniform(0.8, 1.2)))
+    elif theory == "lightly":
+        lat = _lattice(L, 0.15, 0.2)
+        extra = dict(mpi=float(rng.uniform(0.5, 1.5)), alpha=float(rng.uniform(0.0, 0.3)))
     else:
         lat = _lattice(L, 0.15, 0.2)
         extra = dict(kappa=0.1)

@@ -8,7 +8,8 @@ from skyrmion_solver_amd import initial_data
 from skyrmion_solver_amd.theories import load_theory
 
 THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model",
-                "coulomb": "Coulomb-Skyrme model", "rho": "Rho-Skyrme model"}
+                "coulomb": "Coulomb-Skyrme model", "rho": "Rho-Skyrme model",
+                "berger": "Berger Skyrme model", "lightly": "Lightly bound Skyrme model"}
 
 
 def make_params(theory: str, X: int, Y: int, Z: int, **kw):
@@ -22,6 +23,10 @@ def make_params(theory: str, X: int, Y: int, Z: int, **kw):
         base.update(c_omega=0.5, mass=0.45)
     if theory == "coulomb":
         base.update(kappa=0.3, mass=0.6)
+    if theory == "berger":
+        base.update(mpi=0.7, kappa=1.1, alpha=0.6)  # beta = alpha^2 - 1 = -0.64
+    if theory == "lightly":
+        base.update(mpi=0.5, alpha=0.3)
     if theory == "rho":
         # the rho Laplacian enters with weight 8 (pion: 2): RK4 needs the smaller step
         base.update(mpi=0.5, mrho=1.2, c_alpha=0.3, time_step=0.02)

@@ -18,13 +18,13 @@ from skyrmion_solver_amd import _engine
 pytestmark = pytest.mark.gpu
 
 TID = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
-       "rho": _engine.THEORY_RHO}
+       "rho": _engine.THEORY_RHO, "berger": _engine.THEORY_BERGER, "lightly": _engine.THEORY_LIGHTLY_BOUND}
 
 
 def lattices(theory, shapes, seed0=0):
     out = []
     for k, sh in enumerate(shapes):
-        kw = {"mpi": 0.5 + 0.2 * k} if theory in ("pion", "rho") else {"mass": 0.4 + 0.1 * k}
+        kw = {"mpi": 0.5 + 0.2 * k} if theory in ("pion", "rho", "berger", "lightly") else {"mass": 0.4 + 0.1 * k}
         mod, rp, p_i, p_f = make_params(theory, *sh, **kw)
         F = smooth_field(rp, int(p_i[5]), seed=seed0 + k, normalise=True)
         out.append((p_i, p_f, F))
@@ -40,7 +40,8 @@ def solo(theory, prec, p_i, p_f, F, steps, tol):
 
 @pytest.mark.parametrize("theory,prec", [("pion", _engine.FP64), ("pion", _engine.FP32),
                                          ("coulomb", _engine.FP32), ("omega", _engine.FP64),
-                                         ("rho", _engine.FP64), ("rho", _engine.FP32)])
+                                         ("rho", _engine.FP64), ("rho", _engine.FP32),
+                                         ("berger", _engine.FP32), ("lightly", _engine.FP64)])
 def test_batch_equals_solo_bitwise(theory, prec):
     print(f"[case] batch theory={theory} prec={prec}", flush=True)
     lat = lattices(theory, [(20, 18, 16), (16, 16, 16), (24, 21, 34)])

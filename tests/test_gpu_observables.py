@@ -22,7 +22,7 @@ from skyrmion_solver_amd.core.simulation import Simulation
 pytestmark = pytest.mark.gpu
 
 THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
-              "rho": _engine.THEORY_RHO}
+              "rho": _engine.THEORY_RHO, "berger": _engine.THEORY_BERGER, "lightly": _engine.THEORY_LIGHTLY_BOUND}
 KINDS_ALL = ("com", "rms", "quadrupole", "U", "V", "W", "D")
 SKX_EINVAL = -1
 KIND_IDS = {"com": _engine.OBS_COM, "rms": _engine.OBS_RMS, "quadrupole": _engine.OBS_QUADRUPOLE,
@@ -52,7 +52,7 @@ def check_sums(eng, theory, F, p_i, p_f, kinds):
         assert np.all(err <= 1e-12 * scale + 1e-300), (kind, got, ref, err / np.maximum(scale, 1e-300))
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
 @pytest.mark.parametrize("shape", [(20, 18, 16), (9, 13, 24)])
 def test_observable_sums_fp64(theory, shape):
     _, rp, p_i, p_f = make_params(theory, *shape)
@@ -109,7 +109,7 @@ def test_simulation_observables_pion():
         sim.close()
 
 
-@pytest.mark.parametrize("theory", ["omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["omega", "coulomb", "rho", "berger", "lightly"])
 def test_simulation_observables_other_theories(theory):
     sim, rp, p_i, p_f = _sim(theory, (20, 22, 18))
     try:
@@ -119,7 +119,7 @@ def test_simulation_observables_other_theories(theory):
         np.testing.assert_allclose(sim.compute_center_of_mass(), OO.center_of_mass(theory, F, p_i, p_f),
                                    rtol=1e-11, atol=1e-13)
         assert sim.compute_rms_radius() == pytest.approx(OO.rms_radius(theory, F, p_i, p_f), rel=1e-11)
-        # the reference's omega / Coulomb / rho observables modules have no inertia tensors or D-term
+        # the reference's other observables modules have no inertia tensors or D-term
         with pytest.raises(AttributeError):
             sim.compute_U_tensor()
         with pytest.raises(AttributeError):

@@ -19,7 +19,7 @@ from skyrmion_solver_amd import _engine
 pytestmark = pytest.mark.gpu
 
 THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB,
-              "rho": _engine.THEORY_RHO}
+              "rho": _engine.THEORY_RHO, "berger": _engine.THEORY_BERGER, "lightly": _engine.THEORY_LIGHTLY_BOUND}
 SHAPES = [(20, 18, 16), (9, 7, 13), (24, 40, 70)]
 
 
@@ -28,7 +28,7 @@ def _engine_for(theory, rp, p_i, p_f, prec=_engine.FP64, **kw):
     return _engine.Engine(THEORY_IDS[theory], prec, p_i, p_f, **kw)
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_gradient_probe_fp64(theory, shape):
     mod, rp, p_i, p_f = make_params(theory, *shape)
@@ -58,7 +58,7 @@ def test_gradient_probe_tiles(tile_z, theory):
         assert rel_l2(eg, EG_ref) < 1e-12, (tile_z, xc)
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
 def test_energy_baryon_fp64(theory):
     mod, rp, p_i, p_f = make_params(theory, 22, 20, 18)
     nf = int(p_i[5])
@@ -78,7 +78,7 @@ def test_energy_baryon_fp64(theory):
     assert rel_l2(db, OI.baryon_density(st)) < 1e-12
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
 def test_anf_steps_fp64(theory):
     mod, rp, p_i, p_f = make_params(theory, 20, 18, 16)
     nf = int(p_i[5])
@@ -123,7 +123,7 @@ def test_arrest_applied_like_reference():
     print("arrests:", arrests)
 
 
-@pytest.mark.parametrize("theory", ["pion", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "coulomb", "rho", "berger"])
 def test_run_equals_steps_bitwise(theory):
     """The device-resident loop (fused energy, deferred arrest, hipGraph) equals repeated step()."""
     mod, rp, p_i, p_f = make_params(theory, 18, 17, 20, time_step=0.04)
@@ -157,7 +157,7 @@ def test_run_no_graph_equals_graph():
         assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
 def test_fp32_form_against_fp64_oracle(theory):
     """FP32 storage/stencils: after 10 steps, |dE|/E <= 1e-5, |dB| <= 1e-4, field rel-L2 <= 1e-5."""
     mod, rp, p_i, p_f = make_params(theory, 20, 18, 16)

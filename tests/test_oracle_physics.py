@@ -58,6 +58,12 @@ def params(theory):
     elif theory in ("omega", "coulomb"):
         p_i[5], p_i[11] = 5, 4
         p_f[8], p_f[9] = 0.6, 0.45
+    elif theory == "berger":
+        p_i[5] = 4
+        p_f[8], p_f[9], p_f[12] = 0.7, 1.1, 0.6  # alpha = 0.6: beta = -0.64
+    elif theory == "lightly":
+        p_i[5] = 4
+        p_f[8], p_f[9] = 0.8, 0.3  # alpha = 0.3
     else:
         p_i[5] = 13
         p_f[8], p_f[9], p_f[10] = 0.5, 1.2, 0.3
@@ -175,6 +181,39 @@ def test_rho_gradient_matches_euler_lagrange_of_energy():
     np.testing.assert_allclose(EG, project(g_ref, phi), rtol=1e-10, atol=1e-9)
 
 
+@pytest.mark.parametrize("theory", ["berger", "lightly"])
+def test_berger_and_lightly_bound_gradients_are_euler_lagrange(theory):
+    """Both are plain gradient flows of their coded energies
+    (theories/berger_skyrme/kernels.py:251-337 vs :554-737;
+    theories/lightly_bound_skyrme/kernels.py:275-345 vs :567-683)."""
+    p_i, p_f = params(theory)
+    phi, F, S = random_point_data(4, seed=6 + len(theory))
+    g_ref = euler_lagrange(lambda p, f: OT.ENERGIES[theory](p, f, p_i, p_f), phi, F, S)
+    EG, l = OT.GRADIENTS[theory](phi, F, S, p_i, p_f)
+    np.testing.assert_allclose(EG, project(g_ref, phi), rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(l, -p_f[7] * EG, rtol=0, atol=0)
+
+
+def test_berger_at_alpha_one_is_the_nuclear_model():
+    """alpha = 1 (beta = 0) recovers the standard model (berger kernels.py:275)."""
+    p_i, p_f = params("berger")
+    p_f[12] = 1.0
+    phi, F, S = random_point_data(4, seed=9)
+    np.testing.assert_array_equal(OT.energy_berger(phi, F, p_i, p_f), OT.energy_pion(phi, F, p_i, p_f))
+    np.testing.assert_allclose(OT.gradient_berger(phi, F, S, p_i, p_f)[0], OT.gradient_pion(phi, F, S, p_i, p_f)[0],
+                               rtol=1e-13, atol=1e-13)
+
+
+def test_berger_squashing_terms_bite():
+    """A dropped squashing term (beta) in the gradient fails the identity."""
+    p_i, p_f = params("berger")
+    phi, F, S = random_point_data(4, seed=10)
+    g_ref = project(euler_lagrange(lambda p, f: OT.energy_berger(p, f, p_i, p_f), phi, F, S), phi)
+    q = p_f.copy()
+    q[12] = 1.0  # the gradient without the squashing terms
+    assert np.max(np.abs(OT.gradient_berger(phi, F, S, p_i, q)[0] - g_ref)) > 1e-2
+
+
 def test_planted_coupling_error_fails_the_identity():
     """A 1% error in the quartic coupling of the oracle gradient must fail the identity."""
     p_i, p_f = params("pion")
@@ -186,7 +225,7 @@ def test_planted_coupling_error_fails_the_identity():
     assert np.max(np.abs(EG_bad - project(g_ref, phi))) > 1e-3
 
 
-@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
 def test_vacuum_is_a_fixed_point(theory):
     p_i, p_f = params(theory)
     nf = int(p_i[5])

@@ -17,9 +17,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_registry_names_and_aliases():
     assert set(list_theories()) == {"Nuclear Skyrme model", "Omega-Skyrme model", "Coulomb-Skyrme model",
-                                    "Rho-Skyrme model"}
+                                    "Rho-Skyrme model", "Berger Skyrme model", "Lightly bound Skyrme model"}
     assert get_theory_spec("skyrmions").name == "Nuclear Skyrme model"
     assert get_theory_spec("  MAXWELL skyrme ").name == "Coulomb-Skyrme model"
+    assert get_theory_spec("squashed sphere skyrme").name == "Berger Skyrme model"
+    assert get_theory_spec("Loosely bound Skyrme model").name == "Lightly bound Skyrme model"
     with pytest.raises(KeyError):
         get_theory_spec("baby skyrme")
 
@@ -37,6 +39,18 @@ def test_core_packing_layout():
     assert p_f[6] == p_f[3] * p_f[4] * p_f[5]
     assert p_f[7] == 0.5 * p_f[3]
     assert p_f[8] == 0.3 and p_f[9] == 1.7
+
+
+def test_berger_and_lightly_bound_packing():
+    """p_f[8..12] = mpi, kappa, B, isospin_rotation, alpha (theories/berger_skyrme/params.py:237-243);
+    p_f[8..11] = mpi, alpha, B, isospin_rotation (theories/lightly_bound_skyrme/params.py:234-239)."""
+    be = load_theory("Berger Skyrme model").params
+    p_i, p_f = be.pack_device_params(be.default_params(mpi=0.4, kappa=1.3, alpha=0.7, baryon_number=2.0,
+                                                       isospin_rotation=0.5, ansatz="rational_map").resolved())
+    assert list(p_f[8:]) == [0.4, 1.3, 2.0, 0.5, 0.7] and list(p_i[11:]) == [0, 1, 0] and p_i[5] == 4
+    lb = load_theory("Lightly bound Skyrme model").params
+    p_i, p_f = lb.pack_device_params(lb.default_params(mpi=0.6, alpha=0.2, baryon_number=3.0).resolved())
+    assert list(p_f[8:]) == [0.6, 0.2, 3.0, 0.0] and list(p_i[11:]) == [1, 0, 0] and p_i[5] == 4
 
 
 def test_theory_derived_couplings():
