@@ -421,26 +421,37 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         } else {
           // RK4 combine (core/integrator.py:170-175), recomputing k1..k4 from V and l1..l3,
           // in the storage precision: the result is rounded to it when stored
+          // (FP64 form: the reference's operand order; FP32 form: the same sum regrouped as
+          // F + dt (V + (l1 + l2 + l3) / 6), since k1 + 2 k2 + 2 k3 + k4 = dt (6 V + l1 + l2 + l3))
           using R = T;
           R Fn[NF], Vn[NF];
           const R rdt = R(A.P.dt);
+          // signed max and max |g| of this site in the storage precision: the
+          // conversion to FP64 is exact and monotone, so the FP64 reduction
+          // sees the same values (NaN is ignored by fmax and flagged by `bad`)
+          R gmax = g[0], gabs = fabs(g[0]);
 #pragma unroll
           for (int a = 0; a < NF; ++a) {
             const R Fv = cF[a];
             const R Vk = (keep != T(0)) ? cV[a] : R(0);
             const R l1 = cl1[a], l2 = cl2[a], l3 = cl3[a];
             const R l4 = -dt * g[a];
-            const R k1 = rdt * Vk;
-            const R k2 = rdt * (Vk + R(0.5) * l1);
-            const R k3 = rdt * (Vk + R(0.5) * l2);
-            const R k4 = rdt * (Vk + l3);
             Vn[a] = Vk + R(1.0 / 6.0) * (l1 + R(2.0) * l2 + R(2.0) * l3 + l4);
-            Fn[a] = Fv + R(1.0 / 6.0) * (k1 + R(2.0) * k2 + R(2.0) * k3 + k4);
-            const double gv = double(g[a]);
-            acc0 = fmax(acc0, gv);
-            acc1 = fmax(acc1, fabs(gv));
-            bad |= !isfinite(gv);
+            if constexpr (sizeof(T) == 8) {
+              const R k1 = rdt * Vk;
+              const R k2 = rdt * (Vk + R(0.5) * l1);
+              const R k3 = rdt * (Vk + R(0.5) * l2);
+              const R k4 = rdt * (Vk + l3);
+              Fn[a] = Fv + R(1.0 / 6.0) * (k1 + R(2.0) * k2 + R(2.0) * k3 + k4);
+            } else {
+              Fn[a] = Fv + rdt * (Vk + R(1.0 / 6.0) * (l1 + l2 + l3));
+            }
+            gmax = fmax(gmax, g[a]);
+            gabs = fmax(gabs, fabs(g[a]));
+            bad |= !isfinite(g[a]);
           }
+          acc0 = fmax(acc0, double(gmax));
+          acc1 = fmax(acc1, double(gabs));
           if (A.unit_length) {
             // S^3 renormalisation of F and tangent projection of V in FP64 in both forms
             // (compute_norm_skyrme_field / project_orthogonal_skyrme_field,
@@ -452,15 +463,19 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
               v[a] = double(Vn[a]);
               s += f[a] * f[a];
             }
-            s = sqrt(s);
             if (s != 0.0) {
-              if constexpr (sizeof(T) == 8) {  // FP64 form: one division per component, as the reference
+              if constexpr (sizeof(T) == 8) {  // FP64 form: sqrt, then one division per component, as the reference
+                s = sqrt(s);
 #pragma unroll
                 for (int a = 0; a < NSK; ++a) f[a] /= s;
-              } else {  // FP32 form: the result is rounded to FP32 on store
-                const double inv = 1.0 / s;
+              } else {
+                // FP32 form (the result is rounded to FP32 on store): 1/sqrt(s) from
+                // the hardware FP64 reciprocal square root and one Newton step
+                // y <- y (3 - s y^2) / 2, which squares its relative error
+                double y = __builtin_amdgcn_rsq(s);
+                y = y * (1.5 - (0.5 * s) * (y * y));
 #pragma unroll
-                for (int a = 0; a < NSK; ++a) f[a] *= inv;
+                for (int a = 0; a < NSK; ++a) f[a] *= y;
               }
             }
             double lm = 0.0;
