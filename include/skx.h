@@ -295,7 +295,7 @@ int skx_device_info(int32_t device, char* arch, int32_t n, int32_t* cu_count);
  * (core/io.py:52-92) when component_break = 1, output_density_data_dat
  * (:123-160) when ncomp = 1 and component_break = 0.  `data` holds ncomp
  * grids of X*Y*Z doubles in the reference layout; every value is printed as
- * Python's format(v, ".{precision}g") prints it.  No GPU work. */
+ * Python's format(v, ".{precision}g") prints it, precision 0..100.  No GPU work. */
 int skx_write_dat(const char* path, const double* data, int32_t X, int32_t Y, int32_t Z, int32_t ncomp,
                   int32_t precision, int32_t component_break);
 

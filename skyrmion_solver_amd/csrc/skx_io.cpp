@@ -30,19 +30,23 @@ int record_error(int code, const char* msg);  // skx_runtime.hip: sets skx_last_
 
 namespace {
 
+constexpr int kMaxPrecision = 100;
+// sign, 100 digits, point, "e-308" and the terminator fit with room to spare
+constexpr size_t kValueChars = 128;
+
 size_t format_value(char* out, size_t cap, double v, int precision) {
   if (std::isnan(v)) {
     std::memcpy(out, "nan", 3);
     return 3;
   }
   const int n = std::snprintf(out, cap, "%.*g", precision, v);
-  return n > 0 ? (size_t)n : 0;
+  return n > 0 ? std::min((size_t)n, cap - 1) : 0;
 }
 
 // Format rows [r0, r1) of the (component, x, y) row sequence into `buf`.
 void format_rows(std::string& buf, const double* data, long long r0, long long r1, int X, int Y, int Z,
                  int precision, bool component_break) {
-  char tmp[64];
+  char tmp[kValueChars];
   const long long rows_per_comp = (long long)X * Y;
   for (long long r = r0; r < r1; ++r) {
     const double* row = data + r * Z;  // rows are contiguous in the reference layout
@@ -64,8 +68,8 @@ int skx_write_dat(const char* path, const double* data, int32_t X, int32_t Y, in
                   int32_t precision, int32_t component_break) {
   if (!path || (!data && (long long)X * Y * Z * ncomp > 0))
     return skx::record_error(SKX_EINVAL, "null path or data");
-  if (X < 0 || Y < 0 || Z < 0 || ncomp < 0 || precision < 0 || precision > 60)
-    return skx::record_error(SKX_EINVAL, "bad sizes or precision (0..60)");
+  if (X < 0 || Y < 0 || Z < 0 || ncomp < 0 || precision < 0 || precision > kMaxPrecision)
+    return skx::record_error(SKX_EINVAL, "bad sizes or precision (0..100)");
   std::FILE* f = std::fopen(path, "wb");
   if (!f)
     return skx::record_error(SKX_EINVAL, (std::string("cannot open ") + path + ": " + std::strerror(errno)).c_str());

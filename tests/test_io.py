@@ -58,7 +58,7 @@ def special_values(n, seed):
     return v
 
 
-@pytest.mark.parametrize("precision", [1, 3, 6, 16, 17, 32])
+@pytest.mark.parametrize("precision", [1, 3, 6, 16, 17, 32, 60, 100])
 def test_native_writer_matches_oracle_field(tmp_path, precision):
     X, Y, Z, nf = 3, 5, 7, 2
     F = special_values(nf * X * Y * Z, precision)
@@ -89,13 +89,15 @@ def test_native_writer_errors(tmp_path):
     from skyrmion_solver_amd import _engine
 
     with pytest.raises(_engine.SkxError):
+        SIO.output_field_dat(np.zeros(8), str(tmp_path / "p.dat"), 2, 2, 2, 1, precision=101)
+    with pytest.raises(_engine.SkxError):
         SIO.output_field_dat(np.zeros(8), str(tmp_path / "no_such_dir" / "f.dat"), 2, 2, 2, 1)
     with pytest.raises(ValueError):
         SIO.output_field_dat(np.zeros(7), str(tmp_path / "f.dat"), 2, 2, 2, 1)
 
 
 @pytest.mark.parametrize("name", ["Nuclear Skyrme model", "Omega-Skyrme model", "Coulomb-Skyrme model",
-                                  "Rho-Skyrme model"])
+                                  "Rho-Skyrme model", "Berger Skyrme model", "Lightly bound Skyrme model"])
 def test_theory_bundles_match_oracle(tmp_path, name):
     mod = load_theory(name)
     nf = {"Nuclear Skyrme model": 4, "Rho-Skyrme model": 13}.get(name, 5)
