@@ -100,7 +100,7 @@ def test_native_writer_errors(tmp_path):
                                   "Rho-Skyrme model", "Berger Skyrme model", "Lightly bound Skyrme model"])
 def test_theory_bundles_match_oracle(tmp_path, name):
     mod = load_theory(name)
-    nf = {"Nuclear Skyrme model": 4, "Rho-Skyrme model": 13}.get(name, 5)
+    nf = {"Omega-Skyrme model": 5, "Coulomb-Skyrme model": 5, "Rho-Skyrme model": 13}.get(name, 4)
     X, Y, Z = 4, 3, 5
     rng = np.random.default_rng(3)
     F = rng.normal(size=nf * X * Y * Z)
