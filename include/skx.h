@@ -191,7 +191,7 @@ enum skx_profile_kind {
   SKX_PROFILE_GAUSS = 1, /* f = pi exp(-(r / a)^2),  a = profile_param */
   SKX_PROFILE_FIT = 2    /* profilefun(r, B, m0, 0) (initial_config.py:343-442), m0 = profile_param */
 };
-#define SKX_MAX_LUMPS 8
+#define SKX_MAX_LUMPS 16
 
 typedef struct skx_lump {
   int32_t kind;          /* skx_lump_kind */

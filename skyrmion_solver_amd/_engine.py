@@ -33,7 +33,7 @@ DENSITY_ENERGY, DENSITY_BARYON, DENSITY_POTENTIAL, DENSITY_RHO = 0, 1, 2, 3
 FLAG_NO_GRAPH, FLAG_TRACE = 1, 2
 LUMP_HEDGEHOG, LUMP_RATIONAL_MAP = 0, 1
 PROFILE_EXP, PROFILE_GAUSS, PROFILE_FIT = 0, 1, 2
-MAX_LUMPS = 8
+MAX_LUMPS = 16
 OBS_COM, OBS_RMS, OBS_QUADRUPOLE, OBS_U, OBS_V, OBS_W, OBS_D = 0, 1, 2, 3, 4, 5, 6
 OBS_COUNT = (4, 1, 9, 9, 9, 9, 1)
 

@@ -206,8 +206,8 @@ def _profile(r, profile, bint, rp):
         return np.pi * np.exp(-r / float(profile[1]))
     if kind == "gauss":  # reference hedgehog: f = pi exp(-(r/r0)^2)
         return np.pi * np.exp(-(r / float(profile[1])) ** 2)
-    if kind == "fit":  # reference rational-map fits
-        return profilefun(r, bint, 1.0)
+    if kind == "fit":  # reference rational-map fits, ("fit", m0) or "fit" (m0 = 1)
+        return profilefun(r, bint, float(profile[1]) if isinstance(profile, (tuple, list)) else 1.0)
     raise ValueError(f"unknown profile {profile!r}")
 
 
@@ -355,6 +355,8 @@ def lump_specs(rp, config: dict | None = None):
     elif mode == "product":
         lumps = [_lump_spec(rp, _mode(rp, dict(lump, mode=lump.get("mode", "rational_map"))), lump, rng)
                  for lump in cfg.get("lumps", [])]
+    elif mode in ("smorgasbord", "smorgaasbord", "smorg"):
+        lumps = _smorgasbord_specs(rp, cfg, rng)
     else:
         raise ValueError(f"unknown initial-data mode {mode!r}")
     extra = dict(noise=float(cfg.get("noise", 0.0)), seed=int(cfg.get("seed", 0)),
@@ -368,7 +370,9 @@ def build_field(rp, nf: int, config: dict | None = None) -> np.ndarray:
     ``mode``: ``ground`` / ``uniform`` / ``vacuum``; ``hedgehog``;
     ``rational_map``; ``initial`` (the ansatz named by the parameters);
     ``product`` (quaternion product of ``lumps``, each a dict of ansatz
-    options with its own ``mode``).  Options: ``baryon_number``,
+    options with its own ``mode``); ``smorgasbord`` (the reference's
+    randomly placed rational-map lumps: ``count``, ``lump_baryon``,
+    ``profile_m0``, ``position_scale``, ``seed``).  Options: ``baryon_number``,
     ``isospin_rotation``, ``profile`` (``("exp", a)``, ``("gauss", r0)`` or
     ``"fit"``), ``center``, ``jitter`` (uniform centre offset), ``isorotation``
     (``"haar"`` for a random SU(2) isorotation), ``space_rotation``
@@ -389,6 +393,32 @@ def build_field(rp, nf: int, config: dict | None = None) -> np.ndarray:
     if extra["vacuum_band"]:
         set_vacuum_band(F, int(getattr(rp, "halo", 2)))
     return F
+
+
+def _smorgasbord_specs(rp, cfg, rng):
+    """``_initialize_smorgasbord`` (``theories/nuclear_skyrme/initial_config.py:1267-1407``).
+
+    ``count`` rational-map lumps of ``lump_baryon`` each (count defaults to
+    round(|B|)), centres uniform in +-``position_scale`` x the box size about
+    the box centre and then shifted to zero mean, space and isospin rotations
+    ``rotate_xyz`` by three uniform angles each, fitted profile with
+    ``profile_m0``; drawn from ``default_rng(seed)`` in the reference's order
+    (positions, space angles, isospin angles), so a seed gives the
+    reference's lumps.
+    """
+    B = float(cfg.get("baryon_number", getattr(rp, "baryon_number", 1.0)))
+    count = int(cfg.get("count", max(1, int(round(abs(B))))))
+    lump_baryon = int(cfg.get("lump_baryon", 1))
+    m0 = float(cfg.get("profile_m0", 1.0))
+    scale = float(cfg.get("position_scale", 0.18))
+    d = scale * np.array([rp.xsize, rp.ysize, rp.zsize], dtype=np.float64)
+    pos = rng.uniform(low=-d, high=d, size=(count, 3))
+    pos -= pos.mean(axis=0, keepdims=True)
+    alpha = rng.uniform(0.0, 2.0 * np.pi, size=(count, 3))
+    beta = rng.uniform(0.0, 2.0 * np.pi, size=(count, 3))
+    return [dict(mode="rational_map", B=float(lump_baryon), iso=0.0, profile=("fit", m0), center=pos[n],
+                 rotation=_apply(rotate_xyz, *beta[n]), space_rotation=_apply(rotate_xyz, *alpha[n]))
+            for n in range(count)]
 
 
 def _lump_spec(rp, mode, cfg, rng):
@@ -448,7 +478,8 @@ def device_init_desc(rp, config: dict | None = None, omega_cs: float = 0.0):
             prof = ("gauss", max(rmax / (8.0 * max(abs(sp["B"]), 1.0)), 1.0e-6))
         kind = prof[0] if isinstance(prof, (tuple, list)) else prof
         if prof is None or kind == "fit":
-            q.profile, q.profile_param = _engine.PROFILE_FIT, 1.0
+            q.profile = _engine.PROFILE_FIT
+            q.profile_param = float(prof[1]) if isinstance(prof, (tuple, list)) else 1.0
         elif kind == "exp":
             q.profile, q.profile_param = _engine.PROFILE_EXP, float(prof[1])
         elif kind == "gauss":

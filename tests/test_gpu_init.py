@@ -23,7 +23,11 @@ CONFIGS = [
          lumps=[dict(mode="rational_map", baryon_number=2, center=(-0.8, 0.0, 0.0), profile="fit"),
                 dict(mode="hedgehog", center=(0.9, 0.1, 0.0), profile=("gauss", 0.7), isorotation="haar")]),
 ] + [dict(mode="rational_map", baryon_number=B, profile=("exp", 1.2), isorotation="haar", space_rotation="haar",
-          seed=B, isospin_rotation=0.3) for B in range(1, 10)]
+          seed=B, isospin_rotation=0.3) for B in range(1, 10)] + [
+    dict(mode="smorgasbord", baryon_number=5, seed=11, vacuum_band=True),
+    dict(mode="smorgasbord", count=16, lump_baryon=1, profile_m0=0.6, position_scale=0.25, seed=3),
+    dict(mode="smorgasbord", count=3, lump_baryon=2, seed=7),
+]
 
 
 def _device_field(theory, prec, rp, p_i, p_f, cfg, omega_cs=0.0):
