@@ -178,3 +178,30 @@ def test_fp32_form_against_fp64_oracle(theory):
     assert abs(Eg - E_ref) <= 1e-5 * abs(E_ref)
     assert abs(B - B_ref) <= 1e-4
     assert rel_l2(Fg, st.F) <= 1e-5
+
+
+@pytest.mark.parametrize("theory", ["pion", "berger", "lightly"])
+def test_fp32_default_mixed_tiles_equal_uniform_tile(theory):
+    """FP32 lattices with Y >= 32 run stages 0, 1 (pion) and 4 on a 32x16 tile and stages 2-3 on 32x8;
+    the per-site arithmetic does not depend on the tile, so the fields after the same steps are
+    bit-identical to an all-32x8 run (tile_z=32) and the energies agree to reduction order."""
+    mod, rp, p_i, p_f = make_params(theory, 22, 40, 36)
+    F = smooth_field(rp, 4, seed=8, normalise=True)
+    st = OI.OracleState(theory, p_i, p_f, F)
+    with _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32) as a, \
+            _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32, tile_z=32) as b:
+        a.upload(_engine.FIELD, F)
+        b.upload(_engine.FIELD, F)
+        ea, eb = a.energy(), b.energy()
+        assert abs(ea - eb) <= 1e-13 * abs(eb)
+        assert abs(ea - OI.energy(st)) <= 1e-5 * abs(OI.energy(st))
+        ga, _ = a.eval_gradient(F)
+        gb, _ = b.eval_gradient(F)
+        assert np.array_equal(ga, gb)
+        sa = a.anf_run(12, -1.0, 4)
+        sb = b.anf_run(12, -1.0, 4)
+        assert sa.steps == sb.steps == 12
+        assert sa.err_signed == sb.err_signed and sa.err_abs == sb.err_abs
+        assert abs(sa.energy - sb.energy) <= 1e-13 * abs(sb.energy)
+        assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
+        assert np.array_equal(a.download(_engine.VELOCITY), b.download(_engine.VELOCITY))
