@@ -31,6 +31,8 @@ import numpy as np
 from skyrmion_solver_amd import _engine
 
 TILES = (16, 32, 64)
+# single-lattice candidates: 0 is the engine default (a tile per stage), the others force one tile
+ENGINE_TILES = (0,) + TILES
 
 
 def cache_path() -> Path:
@@ -74,7 +76,7 @@ def _vacuum_field(nf: int, N: int) -> np.ndarray:
 
 
 def tune_engine(theory: int, precision: int, p_i, p_f, F=None, steps: int = 16, device: int = 0,
-                use_cache: bool = True, tiles=TILES, xchunks=None) -> dict:
+                use_cache: bool = True, tiles=ENGINE_TILES, xchunks=None) -> dict:
     """Fastest (tile_z, x_chunk) for one lattice; returns ``{"tile_z", "x_chunk", "us_per_step", "table"}``.
 
     `F` is the field timed (default: the vacuum, which runs the same
