@@ -17,7 +17,7 @@ def test_tune_engine_picks_fastest_and_caches(tmp_path, monkeypatch):
     monkeypatch.setenv("SKX_TUNE_CACHE", str(tmp_path / "tune.json"))
     mod, rp, p_i, p_f = make_params("pion", 24, 20, 40)
     r = autotune.tune_engine(_engine.THEORY_PION, _engine.FP32, p_i, p_f, steps=4, xchunks=[0, 8])
-    assert len(r["table"]) == 3 * 2
+    assert len(r["table"]) == len(autotune.ENGINE_TILES) * 2
     best = min(r["table"], key=lambda t: t["us_per_step"])
     assert (r["tile_z"], r["x_chunk"]) == (best["tile_z"], best["x_chunk"])
     again = autotune.tune_engine(_engine.THEORY_PION, _engine.FP32, p_i, p_f, steps=4, xchunks=[0, 8])
