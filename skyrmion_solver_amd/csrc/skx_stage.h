@@ -200,23 +200,24 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
 
   const T* src[3] = {A.F, A.V, A.la};
   T pre[NA][NST][LPT];
+  // Planes outside the lattice load as zeros: the descriptor points at a plane
+  // inside it and every offset is out of range.  Every global load and store
+  // of the march is issued on every path, so the compiler's memory-counter
+  // waits are exact counts: a wait for one plane's loads never also waits for
+  // later loads or for stores still in flight.
   auto issue = [&](int p) {
-    if (p < 0 || p >= X) {  // uniform: planes outside the lattice are zero
+    const bool pin = (p >= 0) && (p < X);
+    const size_t pc = (size_t)min(max(p, 0), X - 1);
+    uint32_t off[LPT];
 #pragma unroll
-      for (int k = 0; k < NA; ++k)
-#pragma unroll
-        for (int a = 0; a < NST; ++a)
-#pragma unroll
-          for (int j = 0; j < LPT; ++j) pre[k][a][j] = T(0);
-      return;
-    }
+    for (int j = 0; j < LPT; ++j) off[j] = pin ? eoff[j] : kOOB;
 #pragma unroll
     for (int k = 0; k < NA; ++k)
 #pragma unroll
       for (int a = 0; a < NST; ++a) {
-        const auto r = plane_rsrc(src[k] + a * N + (size_t)p * YZ, plane_bytes);
+        const auto r = plane_rsrc(src[k] + a * N + pc * YZ, plane_bytes);
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) pre[k][a][j] = bload<T>(r, eoff[j]);
+        for (int j = 0; j < LPT; ++j) pre[k][a][j] = bload<T>(r, off[j]);
       }
   };
 
@@ -272,52 +273,44 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         rawq[rs][2 * NST + a][tid] = pre[NA > 2 ? 2 : 0][a][0];
       }
     }
-    __syncthreads();
-    if (p + 1 <= xe + 1) issue(p + 1);
 
     const int c = p - 2;
     const bool center = (c >= xs) && own;
+    // centre-site loads, issued before the next plane's loads so that waiting
+    // for them does not wait for those (offsets out of range before the chunk)
+    const uint32_t coff = (c >= xs) ? own_b : kOOB;
+    const size_t pb = (size_t)max(c, 0) * YZ;
     // stage 4: its own site's F, V, l1, l2, l3 for the RK4 combine -- F, V, l2
-    // of the stencil components came from the raw ring above (FP32), the rest
-    // are loaded here (issued early)
+    // of the stencil components came from the raw ring above, the rest are loaded here
     if constexpr (STAGE == 4) {
-      if (c >= xs) {
-        const size_t pb = (size_t)c * YZ;
 #pragma unroll
-        for (int a = 0; a < NF; ++a) {
-          const size_t o = a * N + pb;
-          if (!RAWQ || a >= NST) {
-            cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), own_b);
-            cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), own_b);
-            cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), own_b);
-          }
-          cl1[a] = bload<T>(plane_rsrc(A.lb + o, plane_bytes), own_b);
-          cl3[a] = bload<T>(plane_rsrc(A.lc + o, plane_bytes), own_b);
+      for (int a = 0; a < NF; ++a) {
+        const size_t o = a * N + pb;
+        if (!RAWQ || a >= NST) {
+          cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), coff);
+          cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), coff);
+          cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), coff);
         }
+        cl1[a] = bload<T>(plane_rsrc(A.lb + o, plane_bytes), coff);
+        cl3[a] = bload<T>(plane_rsrc(A.lc + o, plane_bytes), coff);
       }
     }
     // site-only components (stages 0-3; stage 4 has them in cF / cV / cl2) and the curl-pass values
     T cP[NA][NF > NST ? NF - NST : 1];
     T cx_aux[NAUX > 0 ? NAUX : 1];
-    if constexpr (NF > NST || NAUX > 0) {
-      if (c >= xs) {
-        const size_t pb = (size_t)c * YZ;
-        if constexpr (NF > NST && STAGE <= 3) {
+    if constexpr (NF > NST && STAGE <= 3) {
 #pragma unroll
-          for (int k = 0; k < NA; ++k)
+      for (int k = 0; k < NA; ++k)
 #pragma unroll
-            for (int a = NST; a < NF; ++a)
-              cP[k][a - NST] = bload<T>(plane_rsrc(src[k] + a * N + pb, plane_bytes), own_b);
-        }
-        if constexpr (NAUX > 0) {
-          // stages >= 1: D^a_j at aux[3a + j]; stages 0 / 1 with the energy fused: R^a_ij at aux[9 + 3a + p]
-          if constexpr (STAGE >= 1) {
-#pragma unroll
-            for (int q = 0; q < NAUX; ++q) cx_aux[q] = bload<T>(plane_rsrc(A.aux + q * N + pb, plane_bytes), own_b);
-          }
-        }
-      }
+        for (int a = NST; a < NF; ++a) cP[k][a - NST] = bload<T>(plane_rsrc(src[k] + a * N + pb, plane_bytes), coff);
     }
+    if constexpr (NAUX > 0 && STAGE >= 1) {
+      // D^a_j at aux[3a + j] (the energy pass reads R^a_ij at aux[9 + 3a + p] below)
+#pragma unroll
+      for (int q = 0; q < NAUX; ++q) cx_aux[q] = bload<T>(plane_rsrc(A.aux + q * N + pb, plane_bytes), coff);
+    }
+    __syncthreads();
+    issue(p + 1);  // past the end of the chunk: a plane inside the lattice or zeros, never read
 
     // lead plane p: own-column value and raw first differences along y and z
     const int zs = (p + 9) % ZSLOTS;
@@ -339,6 +332,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
       }
     }
 
+    // what this site stores (l_s for stages 1-3, F' and V' for stage 4), zero off the centre
+    T lv[(D2 && STAGE <= 3) ? NF : 1], Fn[STAGE == 4 ? NF : 1], Vn[STAGE == 4 ? NF : 1];
+#pragma unroll
+    for (int a = 0; a < ((D2 && STAGE <= 3) ? NF : 1); ++a) lv[a] = T(0);
+#pragma unroll
+    for (int a = 0; a < (STAGE == 4 ? NF : 1); ++a) Fn[a] = Vn[a] = T(0);
     if (center) {
       // x guards are uniform over the block for this plane
       const bool gx1 = (c >= halo) && (c < X - halo), gx2 = (c > halo) && (c < X - halo);
@@ -417,14 +416,13 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         }
         if constexpr (STAGE <= 3) {
 #pragma unroll
-          for (int a = 0; a < NF; ++a) bstore<T>(-dt * g[a], plane_rsrc(A.lout + a * N + (size_t)c * YZ, plane_bytes), own_b);
+          for (int a = 0; a < NF; ++a) lv[a] = -dt * g[a];
         } else {
           // RK4 combine (core/integrator.py:170-175), recomputing k1..k4 from V and l1..l3,
           // in the storage precision: the result is rounded to it when stored
           // (FP64 form: the reference's operand order; FP32 form: the same sum regrouped as
           // F + dt (V + (l1 + l2 + l3) / 6), since k1 + 2 k2 + 2 k3 + k4 = dt (6 V + l1 + l2 + l3))
           using R = T;
-          R Fn[NF], Vn[NF];
           const R rdt = R(A.P.dt);
           // signed max and max |g| of this site in the storage precision: the
           // conversion to FP64 is exact and monotone, so the FP64 reduction
@@ -488,12 +486,18 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
               Vn[a] = R(v[a]);
             }
           }
-#pragma unroll
-          for (int a = 0; a < NF; ++a) {
-            bstore<T>(T(Fn[a]), plane_rsrc(A.Fout + a * N + (size_t)c * YZ, plane_bytes), own_b);
-            bstore<T>(T(Vn[a]), plane_rsrc(A.Vout + a * N + (size_t)c * YZ, plane_bytes), own_b);
-          }
         }
+      }
+    }
+    if constexpr (D2 && STAGE <= 3) {
+#pragma unroll
+      for (int a = 0; a < NF; ++a) bstore<T>(lv[a], plane_rsrc(A.lout + a * N + pb, plane_bytes), coff);
+    }
+    if constexpr (STAGE == 4) {
+#pragma unroll
+      for (int a = 0; a < NF; ++a) {
+        bstore<T>(Fn[a], plane_rsrc(A.Fout + a * N + pb, plane_bytes), coff);
+        bstore<T>(Vn[a], plane_rsrc(A.Vout + a * N + pb, plane_bytes), coff);
       }
     }
   };
