@@ -241,13 +241,16 @@ int skx_sync(skx_handle h);
 typedef struct skx_batch* skx_batch_handle;
 
 typedef struct skx_batch_desc {
-  int32_t theory;       /* enum skx_theory (pion, omega or Coulomb) */
+  int32_t theory;       /* enum skx_theory */
   int32_t precision;    /* enum skx_precision */
   int32_t device;
   uint32_t flags;       /* SKX_FLAG_NO_GRAPH */
-  int32_t tile_z;       /* 16, 32 (default, 0) or 64 */
+  int32_t tile_z;       /* 16, 32 or 64: one tile for every stage; 0: a tile per stage as for one
+                           lattice (FP32: 32 x 16 for pion stage 1 and for stage 4 of pion, Berger
+                           and lightly bound; 32 x 8 elsewhere) */
   int32_t max_slots;    /* lattices resident at once */
-  int32_t max_blocks;   /* blocks of every batched launch (sum over lattices) */
+  int32_t max_blocks;   /* blocks of every batched launch (sum over lattices; with a tile
+                           per stage, each of its two block maps holds this many) */
 } skx_batch_desc;
 
 typedef struct skx_slot_status {

@@ -335,7 +335,10 @@ class Engine:
 
 
 class Batch:
-    """Many lattices of one theory / precision advanced by shared launches (``skx_batch_*``)."""
+    """Many lattices of one theory / precision advanced by shared launches (``skx_batch_*``).
+
+    ``tile_z`` 16, 32 or 64 runs every stage on that tile; 0 takes the single-lattice default of a
+    tile per stage (FP32: 32 x 16 where that is faster, 32 x 8 elsewhere)."""
 
     def __init__(self, theory: int, precision: int, max_slots: int = 64, max_blocks: int = 16384,
                  tile_z: int = 32, device: int = 0, flags: int = 0):

@@ -31,7 +31,8 @@ import numpy as np
 from skyrmion_solver_amd import _engine
 
 TILES = (16, 32, 64)
-# single-lattice candidates: 0 is the engine default (a tile per stage), the others force one tile
+# candidates of both tuners: 0 is the engine default (a tile per stage: 32 x 16 for the FP32
+# stages that gain from it, 32 x 8 elsewhere), the others force one tile on every stage
 ENGINE_TILES = (0,) + TILES
 
 
@@ -117,7 +118,9 @@ def chunk_rule(L: int, target: int) -> int:
 
 def batch_blocks(dims, tile_z: int, target: int) -> int:
     """Blocks of a batch of lattices `dims` (X, Y, Z) with the (tile_z x 256/tile_z) tile and the
-    ``chunk_rule(X, target)`` x-chunks."""
+    ``chunk_rule(X, target)`` x-chunks.  tile_z 0 (a tile per stage) counts its base 32 x 8 tile,
+    the larger of its two block maps."""
+    tile_z = tile_z or 32
     ty = 256 // tile_z
     total = 0
     for X, Y, Z in dims:
@@ -127,14 +130,15 @@ def batch_blocks(dims, tile_z: int, target: int) -> int:
 
 
 def tune_batch(theory: int, precision: int, lattices, steps: int = 8, device: int = 0, use_cache: bool = True,
-               tiles=TILES, targets=(0, 64, 32, 16), max_slots: int | None = None) -> dict:
+               tiles=ENGINE_TILES, targets=(0, 64, 32, 16), max_slots: int | None = None) -> dict:
     """Fastest (tile_z, x-chunk target) for running `lattices` (a list of ``(p_i, p_f)``) as one batch.
 
     Returns ``{"tile_z", "target", "site_updates_per_s", "table"}``; the
     x-chunk of each lattice is ``chunk_rule(X, target)``."""
     dims = sorted(tuple(int(v) for v in np.asarray(p_i)[:3]) for p_i, _ in lattices)
     sig = ",".join("x".join(map(str, d)) for d in dims)
-    key = f"batch|{_device_key(device)}|th{theory}|p{precision}|{len(dims)}|{hashlib.sha1(sig.encode()).hexdigest()[:12]}"
+    key = (f"batch|{_device_key(device)}|th{theory}|p{precision}|{len(dims)}|{hashlib.sha1(sig.encode()).hexdigest()[:12]}"
+           f"|t{'-'.join(map(str, tiles))}|c{'-'.join(map(str, targets))}")
     if use_cache:
         hit = _load_cache().get(key)
         if hit:

@@ -82,12 +82,15 @@ struct StageArgs {
 // written to the other); the lattice owns blocks
 // [first_block, first_block + nblocks) of every batched stage launch.
 template <typename T>
+// Block map 0 serves the stages on the batch's base tile, map 1 the stages on
+// the 32 x 16 tile (FP32 per-stage defaults); a lattice holds a run of blocks
+// in each map it uses.
 struct BatchSlot {
   StageArgs<T> args[4][2];
-  int first_block;
-  int nblocks;
+  int first_block[2];
+  int nblocks[2];
   int active;
-  int pad_;
+  int pad_[3];
 };
 
 // Kernel launchers (skx_kernels.hip).  Return hipSuccess or the launch error.
@@ -103,16 +106,16 @@ bool shape_supported(const LaunchShape& s);
 hipError_t stage_blocks_per_cu(int prec_fp32, int theory, int stage, const LaunchShape& s, int* blocks);
 
 hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, const BatchSlot<double>* slots,
-                                  const int* blk_inst, int parity, int grid, hipStream_t st);
+                                  const int* blk_inst, int map, int parity, int grid, hipStream_t st);
 hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, const BatchSlot<float>* slots,
-                                  const int* blk_inst, int parity, int grid, hipStream_t st);
+                                  const int* blk_inst, int map, int parity, int grid, hipStream_t st);
 
 // Per-(precision, theory) launchers, instantiated in skx_stage_<theory>_<precision>.hip.
 template <typename T, int TH>
 hipError_t stage_launch(int stage, const StageArgs<T>& a, const LaunchShape& s, hipStream_t st);
 template <typename T, int TH>
 hipError_t stage_launch_batch(int stage, const LaunchShape& s, const BatchSlot<T>* slots, const int* blk_inst,
-                              int parity, int grid, hipStream_t st);
+                              int map, int parity, int grid, hipStream_t st);
 template <typename T, int TH>
 hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks);
 

@@ -74,26 +74,26 @@ hipError_t launch_stage_f32(int theory, int stage, const StageArgs<float>& a, co
 }
 
 hipError_t launch_stage_batch_f64(int theory, int stage, const LaunchShape& s, const BatchSlot<double>* slots,
-                                  const int* blk_inst, int parity, int grid, hipStream_t st) {
+                                  const int* blk_inst, int map, int parity, int grid, hipStream_t st) {
   switch (theory) {
-    case PION: return stage_launch_batch<double, PION>(stage, s, slots, blk_inst, parity, grid, st);
-    case OMEGA: return stage_launch_batch<double, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
-    case COULOMB: return stage_launch_batch<double, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
-    case RHO: return stage_launch_batch<double, RHO>(stage, s, slots, blk_inst, parity, grid, st);
-    case BERGER: return stage_launch_batch<double, BERGER>(stage, s, slots, blk_inst, parity, grid, st);
-    case LIGHTLY: return stage_launch_batch<double, LIGHTLY>(stage, s, slots, blk_inst, parity, grid, st);
+    case PION: return stage_launch_batch<double, PION>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case OMEGA: return stage_launch_batch<double, OMEGA>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case COULOMB: return stage_launch_batch<double, COULOMB>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case RHO: return stage_launch_batch<double, RHO>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case BERGER: return stage_launch_batch<double, BERGER>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case LIGHTLY: return stage_launch_batch<double, LIGHTLY>(stage, s, slots, blk_inst, map, parity, grid, st);
   }
   return hipErrorInvalidValue;
 }
 hipError_t launch_stage_batch_f32(int theory, int stage, const LaunchShape& s, const BatchSlot<float>* slots,
-                                  const int* blk_inst, int parity, int grid, hipStream_t st) {
+                                  const int* blk_inst, int map, int parity, int grid, hipStream_t st) {
   switch (theory) {
-    case PION: return stage_launch_batch<float, PION>(stage, s, slots, blk_inst, parity, grid, st);
-    case OMEGA: return stage_launch_batch<float, OMEGA>(stage, s, slots, blk_inst, parity, grid, st);
-    case COULOMB: return stage_launch_batch<float, COULOMB>(stage, s, slots, blk_inst, parity, grid, st);
-    case RHO: return stage_launch_batch<float, RHO>(stage, s, slots, blk_inst, parity, grid, st);
-    case BERGER: return stage_launch_batch<float, BERGER>(stage, s, slots, blk_inst, parity, grid, st);
-    case LIGHTLY: return stage_launch_batch<float, LIGHTLY>(stage, s, slots, blk_inst, parity, grid, st);
+    case PION: return stage_launch_batch<float, PION>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case OMEGA: return stage_launch_batch<float, OMEGA>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case COULOMB: return stage_launch_batch<float, COULOMB>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case RHO: return stage_launch_batch<float, RHO>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case BERGER: return stage_launch_batch<float, BERGER>(stage, s, slots, blk_inst, map, parity, grid, st);
+    case LIGHTLY: return stage_launch_batch<float, LIGHTLY>(stage, s, slots, blk_inst, map, parity, grid, st);
   }
   return hipErrorInvalidValue;
 }

@@ -129,15 +129,20 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
 // cuts the y-halo share of every plane from 1.69 to 1.41 loads per site
 // (128^3 pion: stage 1 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us).  Stages 2
 // and 3 keep 32x8, where stage 2 runs 3 waves per SIMD and stage 3 measured
-// faster; Berger's stage 1 would spill at 512 threads.  A forced tile_z,
-// small lattices and batch lattices (one tile per batch) keep the base tile
-// for every stage.
+// faster; Berger's stage 1 would spill at 512 threads.  A forced tile_z and
+// small lattices keep the base tile for every stage; a batch makes the same
+// choice for all its lattices at creation (skx_batch_create).
+bool wide_tile_stage(int theory, int stage) {
+  const bool s4 = stage == 4 && (theory == SKX_THEORY_PION || theory == SKX_THEORY_BERGER ||
+                                 theory == SKX_THEORY_LIGHTLY_BOUND);
+  const bool s01 = stage <= 1 && theory == SKX_THEORY_PION;
+  return s4 || s01;
+}
+constexpr LaunchShape kWideTile{32, 16, 1};
+
 LaunchShape stage_tile(const Lattice& L, const skx_desc* d, const LaunchShape& base, int stage, bool solo) {
   if (!solo || d->tile_z != 0 || L.prec != SKX_FP32 || base.tz != 32 || base.ty != 8 || L.Y < 32) return base;
-  const bool s4 = stage == 4 && (L.theory == SKX_THEORY_PION || L.theory == SKX_THEORY_BERGER ||
-                                 L.theory == SKX_THEORY_LIGHTLY_BOUND);
-  const bool s01 = stage <= 1 && L.theory == SKX_THEORY_PION;
-  return (s4 || s01) ? LaunchShape{32, 16, 1} : base;
+  return wide_tile_stage(L.theory, stage) ? kWideTile : base;
 }
 
 // Planes per block along x for one stage kernel: as many x-chunks as keep
@@ -924,15 +929,18 @@ int skx_sync(skx_handle h) {
 struct skx_batch {
   int theory = 0, prec = 0, device = 0;
   uint32_t flags = 0;
-  LaunchShape tile{};
+  LaunchShape tile{};       // base tile: block map 0
+  LaunchShape stile[5]{};   // tile of RK4 stage 1..4
+  int smap[5]{};            // block map of RK4 stage 1..4 (1: the 32 x 16 tile)
+  int nmaps = 1;
   int max_slots = 0, max_blocks = 0;
   hipStream_t stream = nullptr;
   std::vector<skx_engine*> slot;
-  std::vector<int> first_block, nblocks;
-  std::vector<int> h_blk;
+  std::vector<int> first_block[2], nblocks[2];  // per map, per slot
+  std::vector<int> h_blk[2];                    // per map: block -> slot (-1 idle)
   void* d_slots = nullptr;  // BatchSlot<T>[max_slots]
   void* h_slots = nullptr;  // pinned mirror
-  int* d_blk = nullptr;
+  int* d_blk[2] = {nullptr, nullptr};
   DevStatus* d_status = nullptr;
   DevStatus* h_status = nullptr;
   bool dirty = true;
@@ -955,13 +963,15 @@ void fill_slot(skx_batch* b, int i) {
   for (int stage = 1; stage <= 4; ++stage)
     for (int par = 0; par < 2; ++par) {
       StageArgs<T> a = make_args<T>(e, stage, fl[stage - 1], par);
-      a.nbz = (e->L.Z + b->tile.tz - 1) / b->tile.tz;
-      a.nby = (e->L.Y + b->tile.ty - 1) / b->tile.ty;
+      a.nbz = (e->L.Z + b->stile[stage].tz - 1) / b->stile[stage].tz;
+      a.nby = (e->L.Y + b->stile[stage].ty - 1) / b->stile[stage].ty;
       a.nbx = (e->L.X + a.xchunk - 1) / a.xchunk;
       sl.args[stage - 1][par] = a;
     }
-  sl.first_block = b->first_block[i];
-  sl.nblocks = b->nblocks[i];
+  for (int m = 0; m < b->nmaps; ++m) {
+    sl.first_block[m] = b->first_block[m][i];
+    sl.nblocks[m] = b->nblocks[m][i];
+  }
   sl.active = 1;
 }
 
@@ -973,7 +983,9 @@ int upload_tables(skx_batch* b) {
   }
   HIP_TRY(hipMemcpyAsync(b->d_slots, b->h_slots, slot_bytes(b->prec) * b->max_slots, hipMemcpyHostToDevice,
                          b->stream));
-  HIP_TRY(hipMemcpyAsync(b->d_blk, b->h_blk.data(), sizeof(int) * b->max_blocks, hipMemcpyHostToDevice, b->stream));
+  for (int m = 0; m < b->nmaps; ++m)
+    HIP_TRY(hipMemcpyAsync(b->d_blk[m], b->h_blk[m].data(), sizeof(int) * b->max_blocks, hipMemcpyHostToDevice,
+                           b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->dirty = false;
   return SKX_OK;
@@ -981,13 +993,14 @@ int upload_tables(skx_batch* b) {
 
 hipError_t enqueue_batch_step(skx_batch* b, int parity) {
   for (int stage = 1; stage <= 4; ++stage) {
+    const int m = b->smap[stage];
     hipError_t r = b->prec == SKX_FP64
-                       ? launch_stage_batch_f64(b->theory, stage, b->tile,
-                                                static_cast<const BatchSlot<double>*>(b->d_slots), b->d_blk, parity,
-                                                b->max_blocks, b->stream)
-                       : launch_stage_batch_f32(b->theory, stage, b->tile,
-                                                static_cast<const BatchSlot<float>*>(b->d_slots), b->d_blk, parity,
-                                                b->max_blocks, b->stream);
+                       ? launch_stage_batch_f64(b->theory, stage, b->stile[stage],
+                                                static_cast<const BatchSlot<double>*>(b->d_slots), b->d_blk[m], m,
+                                                parity, b->max_blocks, b->stream)
+                       : launch_stage_batch_f32(b->theory, stage, b->stile[stage],
+                                                static_cast<const BatchSlot<float>*>(b->d_slots), b->d_blk[m], m,
+                                                parity, b->max_blocks, b->stream);
     if (r != hipSuccess) return r;
   }
   return hipSuccess;
@@ -1025,7 +1038,8 @@ void release_batch(skx_batch* b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   if (b->d_slots) (void)hipFree(b->d_slots);
   if (b->h_slots) (void)hipHostFree(b->h_slots);
-  if (b->d_blk) (void)hipFree(b->d_blk);
+  for (int m = 0; m < 2; ++m)
+    if (b->d_blk[m]) (void)hipFree(b->d_blk[m]);
   if (b->d_status) (void)hipFree(b->d_status);
   if (b->h_status) (void)hipHostFree(b->h_status);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1069,12 +1083,21 @@ int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   b->device = d->device;
   b->flags = d->flags;
   b->tile = d->tile_z == 64 ? LaunchShape{64, 4, 1} : (d->tile_z == 16 ? LaunchShape{16, 16, 1} : LaunchShape{32, 8, 1});
+  // tile_z 0: the FP32 per-stage defaults of a single lattice (stage_tile), for every lattice of the batch
+  for (int stage = 1; stage <= 4; ++stage) {
+    const bool wide = d->tile_z == 0 && d->precision == SKX_FP32 && wide_tile_stage(d->theory, stage);
+    b->stile[stage] = wide ? kWideTile : b->tile;
+    b->smap[stage] = wide ? 1 : 0;
+    if (wide) b->nmaps = 2;
+  }
   b->max_slots = d->max_slots;
   b->max_blocks = d->max_blocks;
   b->slot.assign(b->max_slots, nullptr);
-  b->first_block.assign(b->max_slots, 0);
-  b->nblocks.assign(b->max_slots, 0);
-  b->h_blk.assign(b->max_blocks, -1);
+  for (int m = 0; m < 2; ++m) {
+    b->first_block[m].assign(b->max_slots, 0);
+    b->nblocks[m].assign(b->max_slots, 0);
+    b->h_blk[m].assign(b->max_blocks, -1);
+  }
   hipError_t r;
   if ((r = hipSetDevice(b->device)) != hipSuccess) {
     delete b;
@@ -1087,7 +1110,8 @@ int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   }
   const size_t sb = slot_bytes(b->prec) * b->max_slots;
   if (hipMalloc(&b->d_slots, sb) != hipSuccess || hipHostMalloc(&b->h_slots, sb, hipHostMallocDefault) != hipSuccess ||
-      hipMalloc(&b->d_blk, sizeof(int) * b->max_blocks) != hipSuccess ||
+      hipMalloc(&b->d_blk[0], sizeof(int) * b->max_blocks) != hipSuccess ||
+      (b->nmaps > 1 && hipMalloc(&b->d_blk[1], sizeof(int) * b->max_blocks) != hipSuccess) ||
       hipMalloc(&b->d_status, sizeof(DevStatus) * b->max_slots) != hipSuccess ||
       hipHostMalloc(&b->h_status, sizeof(DevStatus) * b->max_slots, hipHostMallocDefault) != hipSuccess) {
     release_batch(b);
@@ -1120,17 +1144,21 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   // x-chunk: the whole lattice unless the caller asks for shorter marches
   // (the batch's other lattices supply the parallelism)
   const int xc = (lattice->x_chunk > 0 && lattice->x_chunk < L.X) ? lattice->x_chunk : L.X;
-  const int nblk = (int)(((L.Z + b->tile.tz - 1) / b->tile.tz) * (long long)((L.Y + b->tile.ty - 1) / b->tile.ty) *
-                         ((L.X + xc - 1) / xc));
-  int first = -1;
-  for (int s = 0, run = 0; s < b->max_blocks; ++s) {  // first fit in the block table
-    run = (b->h_blk[s] < 0) ? run + 1 : 0;
-    if (run == nblk) {
-      first = s - nblk + 1;
-      break;
+  int first[2] = {-1, -1}, nblk[2] = {0, 0};
+  for (int m = 0; m < b->nmaps; ++m) {  // first fit in each block map
+    const LaunchShape& t = m == 0 ? b->tile : kWideTile;
+    const long long nb = (long long)((L.Z + t.tz - 1) / t.tz) * ((L.Y + t.ty - 1) / t.ty) * ((L.X + xc - 1) / xc);
+    if (nb > b->max_blocks) return fail(SKX_EBUSY, "not enough free blocks in the batch grid");
+    nblk[m] = (int)nb;
+    for (int s = 0, run = 0; s < b->max_blocks; ++s) {
+      run = (b->h_blk[m][s] < 0) ? run + 1 : 0;
+      if (run == nblk[m]) {
+        first[m] = s - nblk[m] + 1;
+        break;
+      }
     }
+    if (first[m] < 0) return fail(SKX_EBUSY, "not enough free blocks in the batch grid");
   }
-  if (first < 0) return fail(SKX_EBUSY, "not enough free blocks in the batch grid");
   skx_desc d = *lattice;
   d.tile_z = b->tile.tz;
   skx_engine* e = nullptr;
@@ -1165,9 +1193,11 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
     return rc;
   }
   b->slot[i] = e;
-  b->first_block[i] = first;
-  b->nblocks[i] = nblk;
-  for (int s = first; s < first + nblk; ++s) b->h_blk[s] = i;
+  for (int m = 0; m < b->nmaps; ++m) {
+    b->first_block[m][i] = first[m];
+    b->nblocks[m][i] = nblk[m];
+    for (int s = first[m]; s < first[m] + nblk[m]; ++s) b->h_blk[m][s] = i;
+  }
   b->dirty = true;
   *slot_out = i;
   return SKX_OK;
@@ -1246,7 +1276,8 @@ int skx_batch_remove(skx_batch_handle b, int32_t slot, double* F_out, double* V_
   if (V_out) {
     if (int rc = copy_out(e, V_out, e->V[e->cur], n)) return rc;
   }
-  for (int s = b->first_block[slot]; s < b->first_block[slot] + b->nblocks[slot]; ++s) b->h_blk[s] = -1;
+  for (int m = 0; m < b->nmaps; ++m)
+    for (int s = b->first_block[m][slot]; s < b->first_block[m][slot] + b->nblocks[m][slot]; ++s) b->h_blk[m][s] = -1;
   b->slot[slot] = nullptr;
   release(e);
   std::memset(b->h_status + slot, 0, sizeof(DevStatus));

@@ -580,18 +580,19 @@ k_stage(const StageArgs<T> A) {
 }
 
 // Batched launch: many lattices of one theory / precision / tile shape in one
-// grid.  blk_inst maps every block of the grid to a slot (-1 = idle block);
-// the slot carries the lattice's arguments for both ping-pong parities.
+// grid.  blk_inst (block map `map` of the batch) maps every block of the grid
+// to a slot (-1 = idle block); the slot carries the lattice's arguments for
+// both ping-pong parities and its run of blocks in each map.
 template <typename T, int TH, int STAGE, int TZ, int TY>
 __global__ void __launch_bounds__(TZ* TY) __attribute__((amdgpu_waves_per_eu(stage_min_waves<T, TH, STAGE>())))
-k_stage_batch(const BatchSlot<T>* __restrict__ slots,
-                                                       const int* __restrict__ blk_inst, int parity) {
+k_stage_batch(const BatchSlot<T>* __restrict__ slots, const int* __restrict__ blk_inst, int map, int parity) {
   const int inst = blk_inst[blockIdx.x];
   if (inst < 0) return;
   const BatchSlot<T>& sl = slots[inst];
   if (!sl.active) return;
   if constexpr (STAGE >= 1)
-    stage_body<T, TH, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], (int)blockIdx.x - sl.first_block, sl.nblocks);
+    stage_body<T, TH, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], (int)blockIdx.x - sl.first_block[map],
+                                     sl.nblocks[map]);
 }
 
 // ------------------------------------------------------------------ per-theory launch
@@ -646,7 +647,7 @@ template <typename T, int TH>
 struct StageBatchLaunchFn {
   const BatchSlot<T>* slots;
   const int* blk_inst;
-  int parity, grid;
+  int map, parity, grid;
   hipStream_t st;
   template <int STAGE, int TZ, int TY>
   hipError_t operator()() {
@@ -656,7 +657,7 @@ struct StageBatchLaunchFn {
       const hipError_t r = hipGetLastError();
       if (r != hipSuccess) return r;
     }
-    hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY>), dim3(grid), dim3(TZ * TY), 0, st, slots, blk_inst,
+    hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY>), dim3(grid), dim3(TZ * TY), 0, st, slots, blk_inst, map,
                        parity);
     return hipGetLastError();
   }
@@ -679,8 +680,8 @@ hipError_t stage_launch(int stage, const StageArgs<T>& a, const LaunchShape& s, 
 
 template <typename T, int TH>
 hipError_t stage_launch_batch(int stage, const LaunchShape& s, const BatchSlot<T>* slots, const int* blk_inst,
-                              int parity, int grid, hipStream_t st) {
-  return visit_shape<T, TH>(stage, s, StageBatchLaunchFn<T, TH>{slots, blk_inst, parity, grid, st});
+                              int map, int parity, int grid, hipStream_t st) {
+  return visit_shape<T, TH>(stage, s, StageBatchLaunchFn<T, TH>{slots, blk_inst, map, parity, grid, st});
 }
 
 template <typename T, int TH>
@@ -692,7 +693,7 @@ hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks) {
 #define SKX_INSTANTIATE_STAGES(T, TH)                                                                          \
   template hipError_t stage_launch<T, TH>(int, const StageArgs<T>&, const LaunchShape&, hipStream_t);          \
   template hipError_t stage_launch_batch<T, TH>(int, const LaunchShape&, const BatchSlot<T>*, const int*, int, \
-                                                int, hipStream_t);                                             \
+                                                int, int, hipStream_t);                                        \
   template hipError_t stage_occupancy<T, TH>(int, const LaunchShape&, int*);
 
 }  // namespace skx

@@ -31,6 +31,8 @@ def test_batch_blocks_counts_tiles_and_chunks():
     # 48^3 with 16x16 tiles: 3 x 3 tiles, whole lattice per block
     assert autotune.batch_blocks([(48, 48, 48)], 16, 64) == 9
     assert autotune.batch_blocks([(48, 48, 48), (64, 64, 64)], 64, 0) == 1 * 12 + 1 * 16
+    # a tile per stage: its base 32 x 8 map is the larger one
+    assert autotune.batch_blocks([(128, 128, 128)], 0, 32) == autotune.batch_blocks([(128, 128, 128)], 32, 32)
 
 
 def test_cache_round_trip(tmp_path, monkeypatch):

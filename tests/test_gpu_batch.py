@@ -60,6 +60,38 @@ def test_batch_equals_solo_bitwise(theory, prec):
         assert np.array_equal(Vb, Vs)
 
 
+@pytest.mark.parametrize("theory,prec", [("pion", _engine.FP32), ("berger", _engine.FP32),
+                                         ("lightly", _engine.FP32), ("pion", _engine.FP64)])
+def test_batch_tile_per_stage_equals_solo(theory, prec):
+    """tile_z 0: FP32 stages on 32 x 16 tiles through the second block map (lattices with Y below
+    and above 32), a lattice leaving half-way through and another joining.  Fields and forces are
+    bit-identical to the lattice run alone with its default tiles; the energy agrees to reduction
+    order (a lattice with Y < 32 runs 32 x 8 alone, so its blocks partition the sum differently)."""
+    print(f"[case] batch tile per stage theory={theory} prec={prec}", flush=True)
+    lat = lattices(theory, [(20, 40, 36), (16, 18, 16), (24, 33, 34), (18, 36, 20)], seed0=20)
+    with _engine.Batch(TID[theory], prec, max_slots=4, max_blocks=96, tile_z=0) as b:
+        s0 = b.add(*lat[0], max_steps=32)
+        s1 = b.add(*lat[1], max_steps=16)
+        s2 = b.add(*lat[2], max_steps=32)
+        b.run(16)
+        out = {1: b.remove(s1)}
+        s3 = b.add(*lat[3], max_steps=16)  # takes free blocks of both maps after lattice 1 left
+        b.run(16)
+        out.update({0: b.remove(s0), 2: b.remove(s2), 3: b.remove(s3)})
+    for k, steps in ((0, 32), (1, 16), (2, 32), (3, 16)):
+        p_i, p_f, F = lat[k]
+        with _engine.Engine(TID[theory], prec, p_i, p_f, x_chunk=int(p_i[0])) as e:
+            e.upload(_engine.FIELD, F)
+            ref = e.anf_run(steps, -1.0, 16)
+            Fs, Vs = e.download(_engine.FIELD), e.download(_engine.VELOCITY)
+        st, Fb, Vb = out[k]
+        assert st["steps"] == ref.steps == steps
+        assert abs(st["energy"] - ref.energy) <= 1e-13 * abs(ref.energy)
+        assert st["err_abs"] == ref.err_abs
+        assert np.array_equal(Fb, Fs)
+        assert np.array_equal(Vb, Vs)
+
+
 def test_late_joiner_does_not_perturb_running_lattices():
     print("[case] batch late joiner", flush=True)
     lat = lattices("pion", [(18, 18, 18), (22, 16, 20), (16, 20, 18)], seed0=10)
