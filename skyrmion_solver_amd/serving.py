@@ -67,7 +67,7 @@ class Server:
     """Admission by HBM footprint + continuous batching over ``skx_batch`` handles."""
 
     def __init__(self, precision: int = _engine.FP32, hbm_budget_bytes: int = 64 << 30, chunk_steps: int = 32,
-                 max_slots: int = 64, max_blocks: int = 32768, tile_z: int = 32, device: int = 0,
+                 max_slots: int = 64, max_blocks: int = 32768, tile_z: int = 0, device: int = 0,
                  keep_fields: bool = True, batch_factory=None, footprint_fn=None, clock=time.perf_counter):
         self.precision = precision
         self.budget = int(hbm_budget_bytes)

@@ -34,7 +34,8 @@ def test_served_results_equal_solo(prec):
         p_i, p_f, F = build(th, sh, 40 + k, prec)
         reqs.append(Request(rid=k, theory=th, p_i=p_i, p_f=p_f, F0=F, tol=2.0, max_steps=120 + 10 * k,
                             arrival=0.003 * k))
-    srv = Server(precision=prec, hbm_budget_bytes=1 << 30, chunk_steps=8, max_slots=8, max_blocks=2048)
+    srv = Server(precision=prec, hbm_budget_bytes=1 << 30, chunk_steps=8, max_slots=8, max_blocks=2048,
+                 tile_z=32)  # the solo runs below use the same tile: energies equal bit for bit
     rep = srv.serve(reqs)
     srv.close()
     print("  report", rep, flush=True)
