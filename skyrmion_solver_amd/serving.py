@@ -23,6 +23,7 @@ every site exactly as a solo run does.
 
 from __future__ import annotations
 
+import sys
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -146,12 +147,20 @@ class Server:
     def idle(self) -> bool:
         return not self.queue and not self.running
 
-    def serve(self, requests, poll_sleep: float = 1.0e-4):
-        """Replay `requests` (their `arrival` in seconds from now) until all have finished."""
+    def serve(self, requests, poll_sleep: float = 1.0e-4, progress_every: float = 0.0):
+        """Replay `requests` (their `arrival` in seconds from now) until all have finished.  With
+        `progress_every` > 0, a line on stderr every that many seconds gives the finished, running and
+        queued counts."""
         pending = deque(sorted(requests, key=lambda r: r.arrival))
+        total = len(pending)
         self.t0 = self.clock()
+        next_note = progress_every
         while pending or not self.idle():
             now = self.now()
+            if progress_every > 0 and now >= next_note:
+                print(f"[serve] t={now:.0f}s finished {len(self.done)}/{total} running {len(self.running)} "
+                      f"queued {len(self.queue)}", file=sys.stderr, flush=True)
+                next_note = now + progress_every
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
             if self.idle():
@@ -212,13 +221,14 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--hbm-gib", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines on stderr (0: none)")
     args = ap.parse_args(argv)
     prec = _engine.FP32 if args.precision == "fp32" else _engine.FP64
     for rate in args.rates:
         reqs = workloads.c5_requests(n=args.n, rate=rate, seed=args.seed, sizes=args.sizes, tol=args.tol,
                                      max_steps=args.max_steps, device_init=True)
         srv = Server(precision=prec, hbm_budget_bytes=int(args.hbm_gib * (1 << 30)), chunk_steps=args.chunk)
-        rep = srv.serve(reqs)
+        rep = srv.serve(reqs, progress_every=args.progress)
         srv.close()
         rep.update(rate_per_s=rate, precision=args.precision, sizes=args.sizes, tol=args.tol,
                    max_steps=args.max_steps)

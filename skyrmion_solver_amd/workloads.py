@@ -5,8 +5,12 @@ p_f, F0)``.  The time step is the reference's example setting
 ``courant = 0.2`` (``examples/nuclear_skyrme_gl.py:13``): the larger
 dt = 0.5 dx these workloads were specified with is beyond the RK4 stability
 limit of the reference's own scheme (the run goes non-finite within tens of
-steps).  The rho theory takes ``courant = 0.1``: its Laplacian enters the
-gradient with weight 8 (the pion's with 2), which halves the stable step.
+steps).  The rho theory takes ``courant = 0.05``: its Laplacian enters the
+gradient with weight 8 (the pion's with 2), and the B >= 2 cores of the
+serving requests stiffen it further.  At 0.1, five of the eight rho requests
+of the 48-request serving stream (seed 0) went non-finite within 100-240
+steps, and one at 0.07; at 0.05 all eight stayed finite and converged or kept
+relaxing.
 """
 
 from __future__ import annotations
@@ -17,7 +21,7 @@ from skyrmion_solver_amd import initial_data
 from skyrmion_solver_amd.theories import load_theory
 
 COURANT = 0.2
-COURANT_BY_THEORY = {"pion": 0.2, "omega": 0.2, "coulomb": 0.2, "rho": 0.1, "berger": 0.2, "lightly": 0.2}
+COURANT_BY_THEORY = {"pion": 0.2, "omega": 0.2, "coulomb": 0.2, "rho": 0.05, "berger": 0.2, "lightly": 0.2}
 THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model", "coulomb": "Coulomb-Skyrme model",
                 "rho": "Rho-Skyrme model", "berger": "Berger Skyrme model", "lightly": "Lightly bound Skyrme model"}
 SERVE_MIX = (("pion", 0.60), ("omega", 0.25), ("rho", 0.15))
@@ -112,7 +116,7 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
     * omega: the units of examples/omega_skyrme_gl.py:11-16 (dx = 12/63, courant 0.2, f_pi = 139.8,
       m_pi = 43.91, m_omega = 249.5) with beta_omega ~ U[7.5, 8.5] (c_omega ~ 14); omega starts at
       -c_omega CS;
-    * rho: dx = 0.15, courant 0.1 (the rho Laplacian has weight 8), the physical masses of
+    * rho: dx = 0.15, courant 0.05 (the rho Laplacian has weight 8), the physical masses of
       examples/rho_skyrme_gl.py:12-13 (e = 3.65, f_pi = 129, m_pi = 138, m_rho = 775) with
       alpha ~ U[0.055, 0.068]; rho starts at 0.
 
@@ -127,22 +131,22 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
 
     th = load_theory(THEORY_NAMES[theory])
     if theory == "pion":
-        lat = _lattice(L, 0.15, 0.2)
+        lat = _lattice(L, 0.15, COURANT_BY_THEORY[theory])
         extra = dict(mpi=float(rng.uniform(0.5, 1.5)), kappa=1.0)
     elif theory == "omega":
-        lat = _lattice(L, 12.0 / 63.0, 0.2)
+        lat = _lattice(L, 12.0 / 63.0, COURANT_BY_THEORY[theory])
         extra = dict(f_pi=139.8, m_pi=43.91, m_omega=249.5, beta_omega=float(rng.uniform(7.5, 8.5)))
     elif theory == "rho":
-        lat = _lattice(L, 0.15, 0.1)
+        lat = _lattice(L, 0.15, COURANT_BY_THEORY[theory])
         extra = dict(e_skyrme=3.65, f_pi=129.0, mass_pion=138.0, mass_rho=775.0, alpha=float(rng.uniform(0.055, 0.068)))
     elif theory == "berger":
-        lat = _lattice(L, 0.15, 0.2)
+        lat = _lattice(L, 0.15, COURANT_BY_THEORY[theory])
         extra = dict(mpi=float(rng.uniform(0.5, 1.5)), kappa=1.0, alpha=float(rng.uniform(0.8, 1.2)))
     elif theory == "lightly":
-        lat = _lattice(L, 0.15, 0.2)
+        lat = _lattice(L, 0.15, COURANT_BY_THEORY[theory])
         extra = dict(mpi=float(rng.uniform(0.5, 1.5)), alpha=float(rng.uniform(0.0, 0.3)))
     else:
-        lat = _lattice(L, 0.15, 0.2)
+        lat = _lattice(L, 0.15, COURANT_BY_THEORY[theory])
         extra = dict(kappa=0.1)
     rp = th.params.default_params(**lat, baryon_number=float(B), **extra).resolved()
     p_i, p_f = th.params.pack_device_params(rp)

@@ -116,3 +116,27 @@ def test_request_larger_than_budget_is_rejected():
     srv.submit(make_requests(1)[0])
     with pytest.raises(ValueError):
         srv.step()
+
+
+def test_progress_lines_report_counts(capsys):
+    t = [0.0]
+
+    def clock():  # fake time: every read advances 10 ms
+        t[0] += 0.01
+        return t[0]
+
+    srv = Server(hbm_budget_bytes=10 ** 9, chunk_steps=4, batch_factory=lambda th: FakeBatch(th, 2, []),
+                 footprint_fn=lambda r: 1, clock=clock)
+    reqs = make_requests(6, seed=5)
+    rep = srv.serve(reqs, poll_sleep=0.0, progress_every=0.05)
+    err = capsys.readouterr().err.splitlines()
+    notes = [ln for ln in err if ln.startswith("[serve]")]
+    assert rep["requests"] == 6 and notes
+    for ln in notes:  # finished + running + queued never exceeds the requests that have arrived
+        f = ln.split()
+        done, total = map(int, f[3].split("/"))
+        assert total == 6 and 0 <= done <= 6 and int(f[5]) <= 4 and int(f[7]) <= 6
+    srv2 = Server(hbm_budget_bytes=10 ** 9, chunk_steps=4, batch_factory=lambda th: FakeBatch(th, 2, []),
+                  footprint_fn=lambda r: 1, clock=clock)
+    srv2.serve(make_requests(3, seed=6), poll_sleep=0.0)
+    assert not [ln for ln in capsys.readouterr().err.splitlines() if ln.startswith("[serve]")]
