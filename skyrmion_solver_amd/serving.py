@@ -16,6 +16,12 @@ tolerance) arrive over time.  The server
   while others run join them on the device (continuous batching);
 * reports time-to-converged (finish - arrival) p50 / p99 and throughput.
 
+FP32 refinement (``refine_fp64=True``, FP32 servers): the FP32 form's force
+has a rounding floor near 1e-4 on stiff solutions, so a lattice whose
+max|g| has fallen below ``refine_factor * tol`` leaves its FP32 batch with
+its field and velocity and finishes in an FP64 batch of its theory.  The
+hand-off waits in a queue that is admitted before new requests.
+
 A lattice's result does not depend on what it shares the device with: each
 one owns its buffers, reductions and status, and the batched kernels compute
 every site exactly as a solo run does.
@@ -54,6 +60,8 @@ class Request:
     status: dict = field(default_factory=dict)
     F: np.ndarray | None = None
     V: np.ndarray | None = None
+    steps_fp32: int = 0  # with FP32 refinement: steps taken in the FP32 form before the hand-off
+    refined: bool = False
 
     @property
     def sites(self) -> int:
@@ -69,7 +77,9 @@ class Server:
 
     def __init__(self, precision: int = _engine.FP32, hbm_budget_bytes: int = 64 << 30, chunk_steps: int = 32,
                  max_slots: int = 64, max_blocks: int = 32768, tile_z: int = 0, device: int = 0,
-                 keep_fields: bool = True, batch_factory=None, footprint_fn=None, clock=time.perf_counter):
+                 keep_fields: bool = True, batch_factory=None, footprint_fn=None, clock=time.perf_counter,
+                 refine_fp64: bool = False, refine_factor: float = 3.0, refine_batch_factory=None,
+                 refine_footprint_fn=None):
         self.precision = precision
         self.budget = int(hbm_budget_bytes)
         self.chunk = int(chunk_steps)
@@ -79,11 +89,19 @@ class Server:
             THEORY_IDS[theory], precision, max_slots=max_slots, max_blocks=max_blocks, tile_z=tile_z, device=device))
         self._footprint = footprint_fn or (lambda r: _engine.footprint_bytes(
             THEORY_IDS[r.theory], precision, r.p_i, r.p_f, tile_z=tile_z))
+        self.refine = bool(refine_fp64) and precision == _engine.FP32
+        self.refine_factor = float(refine_factor)
+        self._refine_factory = refine_batch_factory or (lambda theory: _engine.Batch(
+            THEORY_IDS[theory], _engine.FP64, max_slots=max_slots, max_blocks=max_blocks, tile_z=tile_z,
+            device=device))
+        self._refine_footprint = refine_footprint_fn or (lambda r: _engine.footprint_bytes(
+            THEORY_IDS[r.theory], _engine.FP64, r.p_i, r.p_f, tile_z=tile_z))
         self.clock = clock
         self.t0 = clock()
-        self.batches: dict = {}
+        self.batches: dict = {}  # (kind, theory) -> batch; kind 0 = the server's form, 1 = FP64 refinement
         self.queue: deque = deque()
-        self.running: dict = {}  # (theory, slot) -> (request, footprint)
+        self.handoff: deque = deque()  # (request, F, V, steps left) waiting for an FP64 slot
+        self.running: dict = {}  # (kind, theory, slot) -> (request, footprint)
         self.used = 0
         self.done: list = []
         self.site_updates = 0
@@ -95,12 +113,35 @@ class Server:
     def submit(self, req: Request):
         self.queue.append(req)
 
-    def _batch(self, theory: str):
-        if theory not in self.batches:
-            self.batches[theory] = self._batch_factory(theory)
-        return self.batches[theory]
+    def _batch(self, theory: str, kind: int = 0):
+        if (kind, theory) not in self.batches:
+            make = self._batch_factory if kind == 0 else self._refine_factory
+            self.batches[(kind, theory)] = make(theory)
+        return self.batches[(kind, theory)]
+
+    def _admit_handoffs(self) -> bool:
+        """Admit FP32 lattices waiting to finish in FP64; False while the head does not fit."""
+        while self.handoff:
+            req, F, V, left = self.handoff[0]
+            fp = self._refine_footprint(req)
+            if self.used + fp > self.budget:
+                return False
+            try:
+                slot = self._batch(req.theory, 1).add(req.p_i, req.p_f, F, V, max_steps=left, tol=req.tol,
+                                                      device=self.device)
+            except _engine.SkxError as e:
+                if e.code == -5:
+                    return False
+                raise
+            self.handoff.popleft()
+            self.running[(1, req.theory, slot)] = (req, fp)
+            self.used += fp
+            self.peak_used = max(self.peak_used, self.used)
+        return True
 
     def admit(self):
+        if not self._admit_handoffs():
+            return
         while self.queue:
             req = self.queue[0]
             fp = self._footprint(req)
@@ -117,35 +158,47 @@ class Server:
                 raise
             self.queue.popleft()
             req.admitted = self.now()
-            self.running[(req.theory, slot)] = (req, fp)
+            self.running[(0, req.theory, slot)] = (req, fp)
             self.used += fp
             self.peak_used = max(self.peak_used, self.used)
 
     def step(self):
         """Admit, advance every resident lattice by one chunk, retire the finished ones."""
         self.admit()
-        active = sorted({t for (t, _) in self.running})
-        for t in active:  # enqueue every theory's chunk first: their streams overlap
-            self.batches[t].launch(self.chunk)
-        for t in active:
-            self.batches[t].wait()
-        for t in active:
-            b = self.batches[t]
+        active = sorted({(k, t) for (k, t, _) in self.running})
+        for key in active:  # enqueue every batch's chunk first: their streams overlap
+            self.batches[key].launch(self.chunk)
+        for key in active:
+            self.batches[key].wait()
+        for k, t in active:
+            b = self.batches[(k, t)]
             for st in b.poll():
-                key = (t, st["slot"])
-                if key not in self.running or not st["done"]:
+                key = (k, t, st["slot"])
+                if key not in self.running:
                     continue
-                req, fp = self.running.pop(key)
-                final, F, V = b.remove(st["slot"], want_fields=self.keep_fields)
-                req.finished = self.now()
-                req.status = final
-                req.F, req.V = F, V
-                self.used -= fp
-                self.site_updates += req.sites * int(final["steps"])
-                self.done.append(req)
+                if st["done"]:
+                    req, fp = self.running.pop(key)
+                    final, F, V = b.remove(st["slot"], want_fields=self.keep_fields)
+                    req.finished = self.now()
+                    self.site_updates += req.sites * int(final["steps"])
+                    if req.refined:  # the FP64 part; report the whole run
+                        final = dict(final, steps_fp32=req.steps_fp32, steps=req.steps_fp32 + int(final["steps"]))
+                    req.status = final
+                    req.F, req.V = F, V
+                    self.used -= fp
+                    self.done.append(req)
+                elif k == 0 and self.refine and st["steps"] > 0 and st["err_abs"] < self.refine_factor * \
+                        self.running[key][0].tol:
+                    req, fp = self.running.pop(key)
+                    final, F, V = b.remove(st["slot"], want_fields=True)
+                    self.used -= fp
+                    req.steps_fp32 = int(final["steps"])
+                    req.refined = True
+                    self.site_updates += req.sites * req.steps_fp32
+                    self.handoff.append((req, F, V, max(req.max_steps - req.steps_fp32, 1)))
 
     def idle(self) -> bool:
-        return not self.queue and not self.running
+        return not self.queue and not self.running and not self.handoff
 
     def serve(self, requests, poll_sleep: float = 1.0e-4, progress_every: float = 0.0):
         """Replay `requests` (their `arrival` in seconds from now) until all have finished.  With
@@ -190,7 +243,8 @@ class Server:
                     converged_p99_s=float(np.percentile(conv, 99)) if conv.size else None,
                     makespan_s=float(span), site_updates_per_s=self.site_updates / span if span > 0 else 0.0,
                     converged=int(sum(int(bool(r.status.get("converged"))) for r in self.done)),
-                    by_theory=by, peak_hbm_bytes=self.peak_used)
+                    by_theory=by, peak_hbm_bytes=self.peak_used,
+                    refined=int(sum(int(r.refined) for r in self.done)))
 
     def close(self):
         for b in self.batches.values():
@@ -222,16 +276,20 @@ def main(argv=None):
     ap.add_argument("--hbm-gib", type=float, default=200.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--progress", type=float, default=30.0, help="seconds between progress lines on stderr (0: none)")
+    ap.add_argument("--refine", action="store_true",
+                    help="FP32: finish each lattice in FP64 once its max|g| is below --refine-factor x tol")
+    ap.add_argument("--refine-factor", type=float, default=3.0)
     args = ap.parse_args(argv)
     prec = _engine.FP32 if args.precision == "fp32" else _engine.FP64
     for rate in args.rates:
         reqs = workloads.c5_requests(n=args.n, rate=rate, seed=args.seed, sizes=args.sizes, tol=args.tol,
                                      max_steps=args.max_steps, device_init=True)
-        srv = Server(precision=prec, hbm_budget_bytes=int(args.hbm_gib * (1 << 30)), chunk_steps=args.chunk)
+        srv = Server(precision=prec, hbm_budget_bytes=int(args.hbm_gib * (1 << 30)), chunk_steps=args.chunk,
+                     refine_fp64=args.refine, refine_factor=args.refine_factor)
         rep = srv.serve(reqs, progress_every=args.progress)
         srv.close()
         rep.update(rate_per_s=rate, precision=args.precision, sizes=args.sizes, tol=args.tol,
-                   max_steps=args.max_steps)
+                   max_steps=args.max_steps, refine=bool(args.refine and prec == _engine.FP32))
         print(json.dumps(rep), flush=True)
 
 

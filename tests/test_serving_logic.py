@@ -140,3 +140,68 @@ def test_progress_lines_report_counts(capsys):
                   footprint_fn=lambda r: 1, clock=clock)
     srv2.serve(make_requests(3, seed=6), poll_sleep=0.0)
     assert not [ln for ln in capsys.readouterr().err.splitlines() if ln.startswith("[serve]")]
+
+
+class FloorBatch:
+    """FP32 stand-in: max|g| halves every chunk but never goes below 1.5 tol (a rounding floor);
+    FP64 stand-in (`floor=False`): halves with no floor and converges below tol."""
+
+    def __init__(self, floor, log):
+        self.floor, self.log, self.lat = floor, log, {}
+
+    def add(self, p_i, p_f, F0, V0, max_steps, tol, device, init=None):
+        s = min(set(range(8)) - set(self.lat))
+        err = float(F0[1]) if F0 is not None and not self.floor else 1.0
+        self.lat[s] = dict(rid=int(p_i[1]), err=err, tol=tol, steps=0, max=max_steps, F0=F0, V0=V0)
+        self.log.append(("add", "fp32" if self.floor else "fp64", int(p_i[1]), None if F0 is None else F0.copy()))
+        return s
+
+    def launch(self, steps):
+        for v in self.lat.values():
+            if v["err"] >= v["tol"] and v["steps"] < v["max"]:
+                v["steps"] += steps
+                v["err"] = max(v["err"] / 2, 1.5 * v["tol"]) if self.floor else v["err"] / 2
+
+    def wait(self):
+        pass
+
+    def poll(self):
+        return [dict(slot=s, steps=v["steps"], err_abs=v["err"],
+                     done=int(v["err"] < v["tol"] or v["steps"] >= v["max"])) for s, v in self.lat.items()]
+
+    def remove(self, slot, want_fields=True):
+        v = self.lat.pop(slot)
+        F = np.array([v["rid"], v["err"], 7.0])  # the state handed on: rid, the force reached
+        return dict(steps=v["steps"], converged=int(v["err"] < v["tol"]), energy=1.0), F, F * 2
+
+
+@pytest.mark.parametrize("refine", [True, False])
+def test_fp32_lattices_finish_in_fp64(refine):
+    log = []
+    t = [0.0]
+
+    def clock():
+        t[0] += 0.001
+        return t[0]
+
+    srv = Server(precision=_engine.FP32, hbm_budget_bytes=10 ** 9, chunk_steps=4, footprint_fn=lambda r: 1,
+                 batch_factory=lambda th: FloorBatch(True, log), refine_fp64=refine, refine_factor=3.0,
+                 refine_batch_factory=lambda th: FloorBatch(False, log), refine_footprint_fn=lambda r: 2, clock=clock)
+    reqs = make_requests(5, seed=7)
+    for r in reqs:
+        r.tol, r.max_steps = 1.0e-4, 400
+    rep = srv.serve(reqs, poll_sleep=0.0)
+    assert rep["requests"] == 5
+    if not refine:
+        assert rep["refined"] == 0 and rep["converged"] == 0
+        assert all(r.status["steps"] == 400 for r in reqs)
+        assert not [e for e in log if e[1] == "fp64"]
+        return
+    assert rep["refined"] == 5 and rep["converged"] == 5
+    for r in reqs:
+        # handed off once its force fell below 3 tol: 1 -> 2^-12 < 3e-4 after 12 chunks of 4 steps
+        assert r.refined and r.steps_fp32 == 48
+        assert r.status["steps_fp32"] == 48 and r.status["steps"] > 48 and r.status["converged"]
+        add64 = [e for e in log if e[1] == "fp64" and e[2] == r.rid]
+        assert len(add64) == 1 and add64[0][3][0] == r.rid  # the FP32 field went on to the FP64 batch
+    assert srv.used == 0
