@@ -200,13 +200,29 @@ class Simulation:
         self.last_max_force = err_abs
         return e, err
 
-    def minimize(self, tol: float = 1.0e-4, max_steps: int = 100000, check_every: int = 0) -> dict:
+    def minimize(self, tol: float = 1.0e-4, max_steps: int = 100000, check_every: int = 0,
+                 refine_fp64: bool = False, refine_factor: float = 3.0) -> dict:
         """Run ANF on the device until the stage-4 ``max |g| < tol``.
 
         The reference documents ``minimize(tol=...)`` (``core/simulation.py:8,43``)
         but has no such method; this is the device-resident loop of ``step``.
+
+        ``refine_fp64`` (FP32 simulations): the FP32 form's force has a rounding
+        floor near 1e-4 on stiff solutions, so the run goes in FP32 until
+        ``max |g| < refine_factor * tol`` and then finishes in FP64 from the same
+        field and velocity; the simulation stays in FP64 afterwards.  The result
+        then carries ``steps_fp32`` and ``refined``.
         """
-        return self.engine.anf_run(max_steps, tol, check_every).as_dict()
+        if not refine_fp64 or self.precision != "fp32":
+            return self.engine.anf_run(max_steps, tol, check_every).as_dict()
+        first = self.engine.anf_run(max_steps, refine_factor * tol, check_every).as_dict()
+        if first["err_abs"] < tol or first["steps"] >= max_steps:
+            return dict(first, steps_fp32=first["steps"], refined=False)
+        self.precision = "fp64"
+        self.set_params(self.p)  # a fresh FP64 handle; field and velocity carried over
+        rest = self.engine.anf_run(max_steps - first["steps"], tol, check_every).as_dict()
+        return dict(rest, steps=first["steps"] + rest["steps"], steps_fp32=first["steps"], refined=True,
+                    seconds=first["seconds"] + rest["seconds"])
 
     def save_output(self, output_dir: str | None = None, precision: int = 32, binary: bool = False):
         """Write the theory's output bundle (``core/simulation.py:310-370``).
