@@ -170,15 +170,17 @@ def batch_sample(n: int, steps: int):
 
 def serving_sample(n: int, rate: float, sizes, max_steps: int):
     """Serving form: a Poisson stream of `n` minimisation requests at `rate` per second (pion 60 %, omega
-    25 %, rho 15 %; L from `sizes`; B = 1..8) served to max|g| < 1e-4 by continuous batching, FP32 and
-    FP64 forms; time-to-converged p50 / p99 and throughput."""
+    25 %, rho 15 %; L from `sizes`; B = 1..8) served to max|g| < 1e-4 by continuous batching, in the FP32
+    and FP64 forms and in FP32 with each lattice finished in FP64 once max|g| < 3 tol; time-to-converged
+    p50 / p99 and throughput."""
     from skyrmion_solver_amd import serving, workloads
 
     out = {}
-    for name, prec in (("fp32", _engine.FP32), ("fp64", _engine.FP64)):
+    for name, prec, refine in (("fp32", _engine.FP32, False), ("fp64", _engine.FP64, False),
+                               ("fp32_refined_fp64", _engine.FP32, True)):
         reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps, device_init=True)
         srv = serving.Server(precision=prec, hbm_budget_bytes=64 << 30, chunk_steps=32, max_slots=64,
-                             max_blocks=32768)
+                             max_blocks=32768, refine_fp64=refine)
         rep = srv.serve(reqs)
         srv.close()
         out[name] = rep
