@@ -188,7 +188,8 @@ class Server:
                     self.used -= fp
                     self.done.append(req)
                 elif k == 0 and self.refine and st["steps"] > 0 and st["err_abs"] < self.refine_factor * \
-                        self.running[key][0].tol:
+                        self.running[key][0].tol and self._refine_footprint(self.running[key][0]) <= self.budget:
+                    # (a lattice whose FP64 footprint exceeds the whole budget stays in FP32)
                     req, fp = self.running.pop(key)
                     final, F, V = b.remove(st["slot"], want_fields=True)
                     self.used -= fp

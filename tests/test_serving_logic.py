@@ -205,3 +205,22 @@ def test_fp32_lattices_finish_in_fp64(refine):
         add64 = [e for e in log if e[1] == "fp64" and e[2] == r.rid]
         assert len(add64) == 1 and add64[0][3][0] == r.rid  # the FP32 field went on to the FP64 batch
     assert srv.used == 0
+
+
+def test_refinement_skips_lattices_whose_fp64_footprint_exceeds_the_budget():
+    log = []
+    t = [0.0]
+
+    def clock():
+        t[0] += 0.001
+        return t[0]
+
+    srv = Server(precision=_engine.FP32, hbm_budget_bytes=10, chunk_steps=4, footprint_fn=lambda r: 6,
+                 batch_factory=lambda th: FloorBatch(True, log), refine_fp64=True,
+                 refine_batch_factory=lambda th: FloorBatch(False, log), refine_footprint_fn=lambda r: 11, clock=clock)
+    reqs = make_requests(2, seed=8)
+    for r in reqs:
+        r.tol, r.max_steps = 1.0e-4, 200
+    rep = srv.serve(reqs, poll_sleep=0.0)  # returns: nothing waits for an FP64 slot that can never fit
+    assert rep["requests"] == 2 and rep["refined"] == 0
+    assert all(r.status["steps"] == 200 for r in reqs)
