@@ -265,7 +265,7 @@ int skx_batch_destroy(skx_batch_handle b);
  * stage-4 max|g| < tol or max_steps.  lattice->x_chunk = planes per block
  * along x (0: the whole lattice, one block per (y, z) tile); lattice->tile_z
  * is ignored (the batch's tile applies).  SKX_EBUSY when no slot / blocks are
- * free. */
+ * free now; SKX_EINVAL when the lattice needs more blocks than max_blocks. */
 int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0, const double* V0, int32_t max_steps,
                   double tol, int32_t* slot);
 /* Advance every active lattice by `steps` ANF steps (rounded up to even).

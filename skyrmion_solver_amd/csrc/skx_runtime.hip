@@ -1148,7 +1148,9 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   for (int m = 0; m < b->nmaps; ++m) {  // first fit in each block map
     const LaunchShape& t = m == 0 ? b->tile : kWideTile;
     const long long nb = (long long)((L.Z + t.tz - 1) / t.tz) * ((L.Y + t.ty - 1) / t.ty) * ((L.X + xc - 1) / xc);
-    if (nb > b->max_blocks) return fail(SKX_EBUSY, "not enough free blocks in the batch grid");
+    if (nb > b->max_blocks)  // can never fit, unlike a grid that is busy now
+      return fail(SKX_EINVAL, "the lattice needs " + std::to_string(nb) + " blocks, more than the batch's max_blocks (" +
+                                  std::to_string(b->max_blocks) + ")");
     nblk[m] = (int)nb;
     for (int s = 0, run = 0; s < b->max_blocks; ++s) {
       run = (b->h_blk[m][s] < 0) ? run + 1 : 0;

@@ -155,6 +155,8 @@ class Server:
             except _engine.SkxError as e:
                 if e.code == -5:  # SKX_EBUSY: no slot / blocks free in this batch now
                     return
+                if e.code == -1:  # SKX_EINVAL: e.g. more blocks than the batch's whole grid
+                    raise ValueError(f"request {req.rid} cannot be admitted: {e}") from e
                 raise
             self.queue.popleft()
             req.admitted = self.now()

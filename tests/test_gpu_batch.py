@@ -137,3 +137,11 @@ def test_batch_rejects_when_full():
         with pytest.raises(_engine.SkxError) as e:
             b.add(*lat[0])
         assert e.value.code == -5
+
+
+def test_batch_rejects_a_lattice_larger_than_its_grid():
+    lat = lattices("pion", [(16, 40, 64)])  # 2 x 5 tiles of 32 x 8 per x-chunk
+    with _engine.Batch(TID["pion"], _engine.FP32, max_slots=2, max_blocks=8, tile_z=32) as b:
+        with pytest.raises(_engine.SkxError) as e:
+            b.add(*lat[0])
+        assert e.value.code == -1  # SKX_EINVAL: it could never fit, unlike a busy grid (SKX_EBUSY)

@@ -224,3 +224,14 @@ def test_refinement_skips_lattices_whose_fp64_footprint_exceeds_the_budget():
     rep = srv.serve(reqs, poll_sleep=0.0)  # returns: nothing waits for an FP64 slot that can never fit
     assert rep["requests"] == 2 and rep["refined"] == 0
     assert all(r.status["steps"] == 200 for r in reqs)
+
+
+def test_request_larger_than_its_batch_grid_is_rejected():
+    class TooBig(FakeBatch):
+        def add(self, *a, **k):
+            raise _engine.SkxError(-1, "skx_batch_add", "the lattice needs 99 blocks")
+
+    srv = Server(hbm_budget_bytes=10 ** 9, batch_factory=lambda t: TooBig(t, 4, []), footprint_fn=lambda r: 1)
+    srv.submit(make_requests(1)[0])
+    with pytest.raises(ValueError):
+        srv.step()
