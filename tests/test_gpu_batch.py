@@ -145,3 +145,46 @@ def test_batch_rejects_a_lattice_larger_than_its_grid():
         with pytest.raises(_engine.SkxError) as e:
             b.add(*lat[0])
         assert e.value.code == -1  # SKX_EINVAL: it could never fit, unlike a busy grid (SKX_EBUSY)
+
+
+def test_block_maps_under_churn():
+    """Ten lattices of random shapes through a batch whose grid holds two or three at a time, tile per
+    stage (two block maps): they join as blocks free up, so first fit reuses and fragments both maps.
+    Every result equals the lattice run alone (energy to reduction order: Y < 32 runs 32 x 8 alone)."""
+    print("[case] batch block maps under churn", flush=True)
+    rng = np.random.default_rng(4)
+    shapes = [(int(rng.integers(14, 22)), int(rng.choice([18, 34, 40])), int(rng.integers(16, 40))) for _ in range(10)]
+    lat = lattices("pion", shapes, seed0=40)
+    steps = [16 + 8 * int(rng.integers(0, 3)) for _ in shapes]
+    results, order = {}, []
+    with _engine.Batch(TID["pion"], _engine.FP32, max_slots=4, max_blocks=24, tile_z=0) as b:
+        queue, live = list(range(len(lat))), {}
+        for _ in range(200):
+            while queue:
+                k = queue[0]
+                try:
+                    s = b.add(*lat[k], max_steps=steps[k])
+                except _engine.SkxError as e:
+                    if e.code == -5:
+                        break
+                    raise
+                live[s] = k
+                order.append(k)
+                queue.pop(0)
+            if not live:
+                break
+            b.run(8)
+            for st in b.poll():
+                if st["done"] and st["slot"] in live:
+                    k = live.pop(st["slot"])
+                    results[k] = b.remove(st["slot"])
+    assert sorted(results) == list(range(len(lat))) and order == list(range(len(lat)))
+    for k, (p_i, p_f, F) in enumerate(lat):
+        with _engine.Engine(TID["pion"], _engine.FP32, p_i, p_f, x_chunk=int(p_i[0])) as e:
+            e.upload(_engine.FIELD, F)
+            ref = e.anf_run(steps[k], -1.0, 8)
+            Fs = e.download(_engine.FIELD)
+        st, Fb, _ = results[k]
+        assert st["steps"] == ref.steps == steps[k], k
+        assert abs(st["energy"] - ref.energy) <= 1e-13 * abs(ref.energy), k
+        assert np.array_equal(Fb, Fs), k
