@@ -102,6 +102,35 @@ def test_anf_steps_fp64(theory):
     assert rel_l2(Vg, st.V) < 1e-10 or np.max(np.abs(st.V)) == 0.0
 
 
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "rho", "berger", "lightly"])
+def test_long_trajectory_fp64(theory):
+    """40 steps of the device-resident loop (arrest decided on the device) against 40 oracle steps on a
+    lattice that is no multiple of any tile: every step's energy to 1e-12 of the run's energy scale
+    (norm-wise: on this small box an energy can pass close to zero), the final field to 1e-11."""
+    mod, rp, p_i, p_f = make_params(theory, 14, 12, 16)
+    F = smooth_field(rp, int(p_i[5]), seed=23, normalise=True)
+    st = OI.OracleState(theory, p_i, p_f, F)
+    E = E0 = OI.energy(st)
+    ref, arrests = [], 0
+    for _ in range(40):
+        E, _err = OI.anf_step(st, E)
+        ref.append(E)
+        arrests += int(np.max(np.abs(st.V)) == 0.0)
+    with _engine_for(theory, rp, p_i, p_f) as eng:
+        eng.upload(_engine.FIELD, F)
+        eng.history_attach(48)
+        run = eng.anf_run(40, -1.0, 10)
+        hist = eng.history_read(48)
+        Fg = eng.download(_engine.FIELD)
+    print(f"  {theory}: arrests {arrests}, E0 {hist[0][0]:.12g} -> {run.energy:.12g}", flush=True)
+    assert run.steps == 40 and len(hist) == 40
+    scale = max(abs(E0), max(abs(e) for e in ref))
+    for k in range(39):  # the history holds the energy entering each step
+        assert abs(hist[k + 1][0] - ref[k]) <= 1e-12 * scale, k
+    assert abs(run.energy - ref[-1]) <= 1e-12 * scale
+    assert rel_l2(Fg, st.F) < 1e-11
+
+
 def test_arrest_applied_like_reference():
     """Large dt makes the energy rise: the velocity must be zeroed exactly when E_new > E_prev."""
     mod, rp, p_i, p_f = make_params("pion", 16, 16, 16, time_step=0.04)
