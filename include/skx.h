@@ -246,8 +246,8 @@ typedef struct skx_batch_desc {
   int32_t device;
   uint32_t flags;       /* SKX_FLAG_NO_GRAPH */
   int32_t tile_z;       /* 16, 32 or 64: one tile for every stage; 0: a tile per stage as for one
-                           lattice (FP32: 32 x 16 for pion stage 1 and for stage 4 of pion, Berger
-                           and lightly bound; 32 x 8 elsewhere) */
+                           lattice (FP32: 32 x 16 for the stages that run faster on it, 32 x 8
+                           elsewhere) */
   int32_t max_slots;    /* lattices resident at once */
   int32_t max_blocks;   /* blocks of every batched launch (sum over lattices; with a tile
                            per stage, each of its two block maps holds this many) */

@@ -132,11 +132,20 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
 // faster; Berger's stage 1 would spill at 512 threads.  A forced tile_z and
 // small lattices keep the base tile for every stage; a batch makes the same
 // choice for all its lattices at creation (skx_batch_create).
+//
+// Omega and Coulomb (5 components, 128^3 FP32): their stages 1, 3 and 4 need
+// 256 VGPRs and run 1 wave per SIMD on 32x8, while a 32x16 block holds 2 with
+// some spilling.  Measured per stage (32x8 -> 32x16, us): omega energy pass
+// 48 -> 35, stage 1 57 -> 52, stage 3 64 -> 59, but stage 4 91 -> 139 (204 B
+// spilled) and stage 2 42 -> 44; Coulomb stage 1 80 -> 70, stage 4 111 -> 88,
+// but stages 2-3 slower.  Each takes the faster tile.
 bool wide_tile_stage(int theory, int stage) {
   const bool s4 = stage == 4 && (theory == SKX_THEORY_PION || theory == SKX_THEORY_BERGER ||
-                                 theory == SKX_THEORY_LIGHTLY_BOUND);
-  const bool s01 = stage <= 1 && theory == SKX_THEORY_PION;
-  return s4 || s01;
+                                 theory == SKX_THEORY_LIGHTLY_BOUND || theory == SKX_THEORY_COULOMB);
+  const bool s01 = stage <= 1 && (theory == SKX_THEORY_PION || theory == SKX_THEORY_OMEGA ||
+                                  theory == SKX_THEORY_COULOMB);
+  const bool s3 = stage == 3 && theory == SKX_THEORY_OMEGA;
+  return s4 || s01 || s3;
 }
 constexpr LaunchShape kWideTile{32, 16, 1};
 

@@ -209,13 +209,13 @@ def test_fp32_form_against_fp64_oracle(theory):
     assert rel_l2(Fg, st.F) <= 1e-5
 
 
-@pytest.mark.parametrize("theory", ["pion", "berger", "lightly"])
+@pytest.mark.parametrize("theory", ["pion", "berger", "lightly", "omega", "coulomb"])
 def test_fp32_default_mixed_tiles_equal_uniform_tile(theory):
-    """FP32 lattices with Y >= 32 run stages 0, 1 (pion) and 4 on a 32x16 tile and stages 2-3 on 32x8;
-    the per-site arithmetic does not depend on the tile, so the fields after the same steps are
-    bit-identical to an all-32x8 run (tile_z=32) and the energies agree to reduction order."""
+    """FP32 lattices with Y >= 32 run some stages on a 32x16 tile and the others on 32x8 (the runtime's
+    stage_tile); the per-site arithmetic does not depend on the tile, so the fields after the same steps
+    are bit-identical to an all-32x8 run (tile_z=32) and the energies agree to reduction order."""
     mod, rp, p_i, p_f = make_params(theory, 22, 40, 36)
-    F = smooth_field(rp, 4, seed=8, normalise=True)
+    F = smooth_field(rp, int(p_i[5]), seed=8, normalise=True)
     st = OI.OracleState(theory, p_i, p_f, F)
     with _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32) as a, \
             _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32, tile_z=32) as b:
