@@ -233,12 +233,13 @@ __global__ void __launch_bounds__(TZ* TY) k_rho_curl(const StageArgs<T> A) {
 }
 
 // Batched curl pass: the grid is three copies (one per isospin) of the
-// batch's block map 0 (rho stages run on the base tile); a lattice's blocks
-// of isospin a are numbered a * nblocks + (block - first_block), the order
-// rho_curl_body expects.
+// stage's block map `map`; a lattice's blocks of isospin a are numbered
+// a * nblocks + (block - first_block) in that map, the order rho_curl_body
+// expects.
 template <typename T, int STAGE, int TZ, int TY>
 __global__ void __launch_bounds__(TZ* TY) k_rho_curl_batch(const BatchSlot<T>* __restrict__ slots,
-                                                          const int* __restrict__ blk_inst, int parity, int grid) {
+                                                          const int* __restrict__ blk_inst, int map, int parity,
+                                                          int grid) {
   const int iso = (int)blockIdx.x / grid;
   const int tb = (int)blockIdx.x - iso * grid;
   const int inst = blk_inst[tb];
@@ -246,8 +247,8 @@ __global__ void __launch_bounds__(TZ* TY) k_rho_curl_batch(const BatchSlot<T>* _
   const BatchSlot<T>& sl = slots[inst];
   if (!sl.active) return;
   if constexpr (STAGE >= 1)
-    rho_curl_body<T, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], iso * sl.nblocks[0] + (tb - sl.first_block[0]),
-                                    3 * sl.nblocks[0]);
+    rho_curl_body<T, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], iso * sl.nblocks[map] + (tb - sl.first_block[map]),
+                                    3 * sl.nblocks[map]);
 }
 
 }  // namespace skx

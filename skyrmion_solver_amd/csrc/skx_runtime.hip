@@ -123,28 +123,27 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
   return s;
 }
 
-// Default tile of one stage.  In FP32 the 4-component theories run stage 4
-// (and, for pion, the energy pass and stage 1) on a 32x16 tile:
-// those kernels hold 2 waves per SIMD on either tile, and the taller tile
-// cuts the y-halo share of every plane from 1.69 to 1.41 loads per site
-// (128^3 pion: stage 1 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us).  Stages 2
-// and 3 keep 32x8, where stage 2 runs 3 waves per SIMD and stage 3 measured
-// faster; Berger's stage 1 would spill at 512 threads.  A forced tile_z and
-// small lattices keep the base tile for every stage; a batch makes the same
-// choice for all its lattices at creation (skx_batch_create).
+// Default tile of one stage.  In FP32 some stages run on a 32x16 tile, chosen
+// per theory and stage from measurement at 128^3 (wide_tile_stage).  Where a
+// kernel holds 2 waves per SIMD on either tile, the taller tile cuts the
+// y-halo share of every plane from 1.69 to 1.41 loads per site (pion: stage 1
+// 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us); pion stages 2 and 3 keep 32x8,
+// where stage 2 runs 3 waves per SIMD and stage 3 measured faster.  A forced
+// tile_z and small lattices keep the base tile for every stage; a batch makes
+// the same choice for all its lattices at creation (skx_batch_create).
 //
 // Omega and Coulomb (5 components, 128^3 FP32): their stages 1, 3 and 4 need
 // 256 VGPRs and run 1 wave per SIMD on 32x8, while a 32x16 block holds 2 with
 // some spilling.  Measured per stage (32x8 -> 32x16, us): omega energy pass
 // 48 -> 35, stage 1 57 -> 52, stage 3 64 -> 59, but stage 4 91 -> 139 (204 B
 // spilled) and stage 2 42 -> 44; Coulomb stage 1 80 -> 70, stage 4 111 -> 88,
-// but stages 2-3 slower.  Each takes the faster tile.
+// but stages 2-3 slower.  Berger's stage 1 (1 wave on 32x8) 75 -> 56,
+// lightly bound's 54 -> 49; rho's stage 4 (1 wave on 32x8) 350 -> 324 and
+// stage 3 226 -> 222, but its stage 1 218 -> 283.  Each takes the faster tile.
 bool wide_tile_stage(int theory, int stage) {
-  const bool s4 = stage == 4 && (theory == SKX_THEORY_PION || theory == SKX_THEORY_BERGER ||
-                                 theory == SKX_THEORY_LIGHTLY_BOUND || theory == SKX_THEORY_COULOMB);
-  const bool s01 = stage <= 1 && (theory == SKX_THEORY_PION || theory == SKX_THEORY_OMEGA ||
-                                  theory == SKX_THEORY_COULOMB);
-  const bool s3 = stage == 3 && theory == SKX_THEORY_OMEGA;
+  const bool s4 = stage == 4 && theory != SKX_THEORY_OMEGA;
+  const bool s01 = stage <= 1 && theory != SKX_THEORY_RHO;
+  const bool s3 = stage == 3 && (theory == SKX_THEORY_OMEGA || theory == SKX_THEORY_RHO);
   return s4 || s01 || s3;
 }
 constexpr LaunchShape kWideTile{32, 16, 1};

@@ -613,8 +613,8 @@ hipError_t visit_shape(int stage, const LaunchShape& s, Fn&& fn) {
   if (s.tz == 64 && s.ty == 4) return visit_stage<T, TH, 64, 4>(stage, fn);
   if (s.tz == 32 && s.ty == 8) return visit_stage<T, TH, 32, 8>(stage, fn);
   if (s.tz == 16 && s.ty == 16) return visit_stage<T, TH, 16, 16>(stage, fn);
-  // 32x16: the FP32 default of some stages (the runtime's stage_tile); not for rho
-  if constexpr (sizeof(T) == 4 && TH != RHO) {
+  // 32x16: the FP32 default of some stages (the runtime's stage_tile)
+  if constexpr (sizeof(T) == 4) {
     if (s.tz == 32 && s.ty == 16) return visit_stage<T, TH, 32, 16>(stage, fn);
   }
   return hipErrorInvalidValue;
@@ -653,7 +653,7 @@ struct StageBatchLaunchFn {
   hipError_t operator()() {
     if constexpr (TH == RHO) {  // curl pass of every lattice's three isospins first
       hipLaunchKernelGGL((k_rho_curl_batch<T, STAGE, TZ, TY>), dim3(3 * grid), dim3(TZ * TY), 0, st, slots,
-                         blk_inst, parity, grid);
+                         blk_inst, map, parity, grid);
       const hipError_t r = hipGetLastError();
       if (r != hipSuccess) return r;
     }

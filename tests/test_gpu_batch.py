@@ -62,7 +62,8 @@ def test_batch_equals_solo_bitwise(theory, prec):
 
 @pytest.mark.parametrize("theory,prec", [("pion", _engine.FP32), ("berger", _engine.FP32),
                                          ("lightly", _engine.FP32), ("omega", _engine.FP32),
-                                         ("coulomb", _engine.FP32), ("pion", _engine.FP64)])
+                                         ("coulomb", _engine.FP32), ("rho", _engine.FP32),
+                                         ("pion", _engine.FP64)])
 def test_batch_tile_per_stage_equals_solo(theory, prec):
     """tile_z 0: FP32 stages on 32 x 16 tiles through the second block map (lattices with Y below
     and above 32), a lattice leaving half-way through and another joining.  Fields and forces are
