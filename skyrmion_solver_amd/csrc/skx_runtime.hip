@@ -132,19 +132,16 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
 // tile_z and small lattices keep the base tile for every stage; a batch makes
 // the same choice for all its lattices at creation (skx_batch_create).
 //
-// Omega and Coulomb (5 components, 128^3 FP32): their stages 1, 3 and 4 need
-// 256 VGPRs and run 1 wave per SIMD on 32x8, while a 32x16 block holds 2 with
-// some spilling.  Measured per stage (32x8 -> 32x16, us): omega energy pass
-// 48 -> 35, stage 1 57 -> 52, stage 3 64 -> 59, but stage 4 91 -> 139 (204 B
-// spilled) and stage 2 42 -> 44; Coulomb stage 1 80 -> 70, stage 4 111 -> 88,
-// but stages 2-3 slower.  Berger's stage 1 (1 wave on 32x8) 75 -> 56,
-// lightly bound's 54 -> 49; rho's stage 4 (1 wave on 32x8) 350 -> 324 and
-// stage 3 226 -> 222, but its stage 1 218 -> 283.  Each takes the faster tile.
+// Coulomb's stages 1 and 4 (5 components) and Berger's stage 1 need 256 VGPRs
+// and run 1 wave per SIMD on 32x8, while a 32x16 block holds 2 with some
+// spilling (128^3 FP32, us): Coulomb stage 1 80 -> 70, stage 4 111 -> 88;
+// Berger stage 1 75 -> 56; lightly bound stage 1 54 -> 49.  Omega and rho
+// keep 32x8: their 32x16 gains at 128^3 were 2-3%, and in a batch of a few
+// small lattices the taller tile halves the blocks, which cost the serving
+// sample 8%.
 bool wide_tile_stage(int theory, int stage) {
-  const bool s4 = stage == 4 && theory != SKX_THEORY_OMEGA;
-  const bool s01 = stage <= 1 && theory != SKX_THEORY_RHO;
-  const bool s3 = stage == 3 && (theory == SKX_THEORY_OMEGA || theory == SKX_THEORY_RHO);
-  return s4 || s01 || s3;
+  if (theory == SKX_THEORY_OMEGA || theory == SKX_THEORY_RHO) return false;
+  return stage <= 1 || stage == 4;
 }
 constexpr LaunchShape kWideTile{32, 16, 1};
 

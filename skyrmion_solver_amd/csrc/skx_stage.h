@@ -614,7 +614,7 @@ hipError_t visit_shape(int stage, const LaunchShape& s, Fn&& fn) {
   if (s.tz == 32 && s.ty == 8) return visit_stage<T, TH, 32, 8>(stage, fn);
   if (s.tz == 16 && s.ty == 16) return visit_stage<T, TH, 16, 16>(stage, fn);
   // 32x16: the FP32 default of some stages (the runtime's stage_tile)
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4 && TH != OMEGA && TH != RHO) {
     if (s.tz == 32 && s.ty == 16) return visit_stage<T, TH, 32, 16>(stage, fn);
   }
   return hipErrorInvalidValue;

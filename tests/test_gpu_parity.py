@@ -209,7 +209,7 @@ def test_fp32_form_against_fp64_oracle(theory):
     assert rel_l2(Fg, st.F) <= 1e-5
 
 
-@pytest.mark.parametrize("theory", ["pion", "berger", "lightly", "omega", "coulomb", "rho"])
+@pytest.mark.parametrize("theory", ["pion", "berger", "lightly", "coulomb"])
 def test_fp32_default_mixed_tiles_equal_uniform_tile(theory):
     """FP32 lattices with Y >= 32 run some stages on a 32x16 tile and the others on 32x8 (the runtime's
     stage_tile); the per-site arithmetic does not depend on the tile, so the fields after the same steps

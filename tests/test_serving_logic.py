@@ -17,7 +17,7 @@ class FakeBatch:
         self.theory, self.slots, self.log = theory, slots, log
         self.lat = {}
 
-    def add(self, p_i, p_f, F0, V0, max_steps, tol, device, init=None):
+    def add(self, p_i, p_f, F0, V0, max_steps, tol, device, init=None, x_chunk=0):
         if len(self.lat) >= self.slots:
             raise _engine.SkxError(-5, "skx_batch_add", "busy")
         s = min(set(range(self.slots)) - set(self.lat))
@@ -149,7 +149,7 @@ class FloorBatch:
     def __init__(self, floor, log):
         self.floor, self.log, self.lat = floor, log, {}
 
-    def add(self, p_i, p_f, F0, V0, max_steps, tol, device, init=None):
+    def add(self, p_i, p_f, F0, V0, max_steps, tol, device, init=None, x_chunk=0):
         s = min(set(range(8)) - set(self.lat))
         err = float(F0[1]) if F0 is not None and not self.floor else 1.0
         self.lat[s] = dict(rid=int(p_i[1]), err=err, tol=tol, steps=0, max=max_steps, F0=F0, V0=V0)
