@@ -51,31 +51,42 @@ def workload(L: int = 128, dx: float = 0.15, seed: int = 0):
     return rp, p_i, p_f, F
 
 
-def time_form(prec, p_i, p_f, F, steps, warmup, chunk, trace=True):
-    """Timed device-resident run (hipGraph chunks of `chunk` steps).  With
-    `trace`, the same number of steps is then run again from the same field
-    with every stage kernel bracketed by HIP events on the engine's stream
-    (direct launches), giving per-kernel average durations of this workload."""
-    eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
+def time_form(prec, p_i, p_f, F, steps, warmup, chunk, trace=True, theory=None, trace_min_steps=200):
+    """Timed device-resident run (hipGraph chunks of `chunk` steps, capped at `steps` so that every
+    launched step is a real one; both ping-pong parities' graphs are captured before the timed region).
+
+    With `trace`, a traced pass runs first on a second engine: the same warm-up, then max(steps,
+    `trace_min_steps`) steps from the same field with every stage kernel bracketed by HIP events on the
+    engine's stream (direct launches), giving the per-kernel average durations of this workload.  Run
+    right before the timed pass, it also brings the GPU to its sustained clocks, which a short timed run
+    (tens of steps) would otherwise spend ramping up."""
+    theory = _engine.THEORY_PION if theory is None else theory
+    chunk = max(1, min(chunk, steps))
+    out = {}
+    eng = _engine.Engine(theory, prec, p_i, p_f)
     eng.upload(_engine.FIELD, F)
+    if trace:
+        tr = _engine.Engine(theory, prec, p_i, p_f, flags=_engine.FLAG_TRACE)
+        tr.upload(_engine.FIELD, F)
+        tr.anf_run(warmup, -1.0, chunk)
+        tr.trace_reset()
+        tr.anf_run(max(steps, trace_min_steps), -1.0, chunk)
+        ms, cnt = tr.trace_read()
+        out["kernels"] = {STAGE_NAMES[k]: dict(avg_us=1000.0 * ms[k] / cnt[k], launches=int(cnt[k]))
+                          for k in range(5) if cnt[k]}
+        tr.close()
     eng.anf_run(warmup, -1.0, chunk)
+    eng.prepare(chunk)
     eng.sync()
     t0 = time.perf_counter()
     st = eng.anf_run(steps, -1.0, chunk)
     eng.sync()
     dt = time.perf_counter() - t0
-    out = dict(seconds=dt, steps=st.steps, energy=st.energy, err_abs=st.err_abs, baryon=st.baryon)
+    if st.issued != steps or st.steps != steps:
+        raise RuntimeError(f"timed run issued {st.issued} and took {st.steps} steps, expected {steps}")
+    out.update(seconds=dt, steps=st.steps, issued=st.issued, graph_launches=st.graph_launches, energy=st.energy,
+               err_abs=st.err_abs, baryon=st.baryon)
     eng.close()
-    if trace:
-        eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f, flags=_engine.FLAG_TRACE)
-        eng.upload(_engine.FIELD, F)
-        eng.anf_run(warmup, -1.0, chunk)
-        eng.trace_reset()
-        eng.anf_run(steps, -1.0, chunk)
-        ms, cnt = eng.trace_read()
-        out["kernels"] = {STAGE_NAMES[k]: dict(avg_us=1000.0 * ms[k] / cnt[k], launches=int(cnt[k]))
-                          for k in range(5) if cnt[k]}
-        eng.close()
     return out
 
 
@@ -194,7 +205,7 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--chunk", type=int, default=100,
-                    help="steps per captured graph; --steps should be a multiple so no launch is an early exit")
+                    help="steps per captured graph (capped at --steps; a remainder runs as direct launches)")
     ap.add_argument("--no-trace", action="store_true", help="time without the per-kernel HIP events")
     ap.add_argument("--solve-seeds", type=int, default=3)
     ap.add_argument("--solve-max-steps", type=int, default=20000)

@@ -94,6 +94,9 @@ typedef struct skx_run_stats {
   double err_abs;       /* max |g| of the last stage-4 gradient */
   double baryon;        /* baryon number of the final field */
   double seconds;       /* wall time of the run on the host clock */
+  int32_t issued;       /* steps enqueued on the device: never more than max_steps; steps after the
+                           stop test fired are enqueued but skip their work */
+  int32_t graph_launches; /* captured chunks replayed (the remainder runs as direct launches) */
 } skx_run_stats;
 
 /* One record per step of a run (when a history buffer is attached). */
@@ -130,6 +133,13 @@ int skx_anf_step(skx_handle h, double e_prev, double* e_new, double* err_signed,
  * The arrest decision of each step is taken on the device in FP64.
  * check_every = steps between host polls (0 = engine default). */
 int skx_anf_run(skx_handle h, int32_t max_steps, double tol, int32_t check_every, skx_run_stats* stats);
+
+/* Capture the step graphs skx_anf_run replays for this check_every (one per
+ * ping-pong parity), without running a step, so that a later timed or
+ * latency-bound run does no capture.  No reference counterpart (the
+ * reference launches every kernel from the host); for callers that time
+ * or serve runs.  A no-op with SKX_FLAG_NO_GRAPH or SKX_FLAG_TRACE. */
+int skx_anf_prepare(skx_handle h, int32_t check_every);
 
 /* Plain RK4 steps (killkinen forced off), n steps. */
 int skx_rk4_steps(skx_handle h, int32_t n, skx_run_stats* stats);

@@ -39,7 +39,7 @@ OBS_COUNT = (4, 1, 9, 9, 9, 9, 1)
 
 EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
-    "skx_anf_run", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density", "skx_eval_gradient",
+    "skx_anf_run", "skx_anf_prepare", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density", "skx_eval_gradient",
     "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_sync",
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
@@ -70,7 +70,7 @@ class RunStats(C.Structure):
     _fields_ = [
         ("steps", C.c_int32), ("converged", C.c_int32), ("energy", C.c_double),
         ("err_signed", C.c_double), ("err_abs", C.c_double), ("baryon", C.c_double),
-        ("seconds", C.c_double),
+        ("seconds", C.c_double), ("issued", C.c_int32), ("graph_launches", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -129,6 +129,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_download": ([H, C.c_int32, dp, C.c_size_t], C.c_int),
         "skx_anf_step": ([H, C.c_double, dp, dp, dp], C.c_int),
         "skx_anf_run": ([H, C.c_int32, C.c_double, C.c_int32, C.POINTER(RunStats)], C.c_int),
+        "skx_anf_prepare": ([H, C.c_int32], C.c_int),
         "skx_rk4_steps": ([H, C.c_int32, C.POINTER(RunStats)], C.c_int),
         "skx_energy": ([H, dp], C.c_int),
         "skx_baryon": ([H, dp], C.c_int),
@@ -273,6 +274,11 @@ class Engine:
         _check(self._lib.skx_anf_run(self._h, int(max_steps), float(tol), int(check_every), C.byref(st)),
                "skx_anf_run")
         return st
+
+    def prepare(self, check_every: int = 0):
+        """Capture the step graphs of ``anf_run(..., check_every)`` for both ping-pong parities now, so that
+        a later run replays them without capturing."""
+        _check(self._lib.skx_anf_prepare(self._h, int(check_every)), "skx_anf_prepare")
 
     def rk4_steps(self, n: int) -> RunStats:
         st = RunStats()

@@ -145,8 +145,8 @@ bool wide_tile_stage(int theory, int stage) {
 }
 constexpr LaunchShape kWideTile{32, 16, 1};
 
-LaunchShape stage_tile(const Lattice& L, const skx_desc* d, const LaunchShape& base, int stage, bool solo) {
-  if (!solo || d->tile_z != 0 || L.prec != SKX_FP32 || base.tz != 32 || base.ty != 8 || L.Y < 32) return base;
+LaunchShape stage_tile(const Lattice& L, const LaunchShape& base, int stage, bool per_stage) {
+  if (!per_stage || L.prec != SKX_FP32 || base.tz != 32 || base.ty != 8 || L.Y < 32) return base;
   return wide_tile_stage(L.theory, stage) ? kWideTile : base;
 }
 
@@ -367,6 +367,22 @@ int build_graph(skx_engine* e, int parity, int steps, int fuse_baryon) {
   return SKX_OK;
 }
 
+// Step graphs of `chunk` steps for both starting parities (captured once per
+// chunk size and history setting; a run replays them for whole chunks only).
+int ensure_graphs(skx_engine* e, int chunk, int fuse_baryon) {
+  if (e->graph_steps != chunk || e->graph_flags != fuse_baryon) {
+    drop_graphs(e);
+    e->graph_steps = chunk;
+    e->graph_flags = fuse_baryon;
+  }
+  for (int parity = 0; parity < 2; ++parity)
+    if (!e->graph[parity])
+      if (int rc = build_graph(e, parity, chunk, fuse_baryon)) return rc;
+  return SKX_OK;
+}
+
+bool graphs_enabled(const skx_engine* e) { return !(e->flags & (SKX_FLAG_NO_GRAPH | SKX_FLAG_TRACE)); }
+
 int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, int killkinen, skx_run_stats* out) {
   auto t0 = std::chrono::steady_clock::now();
   DevStatus* h = e->st_host;
@@ -378,28 +394,26 @@ int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, 
   h->hist_cap = e->hist ? e->hist_cap : 0;
   if (int rc = push_status(e)) return rc;
   const int c0 = e->cur;
-  int chunk = check_every > 0 ? check_every : 32;
+  const int chunk = check_every > 0 ? check_every : 32;
   // traced runs launch directly: every kernel is bracketed by its own event pair
-  const bool use_graph = !(e->flags & (SKX_FLAG_NO_GRAPH | SKX_FLAG_TRACE));
+  const bool use_graph = graphs_enabled(e);
   const int fuse_baryon = e->hist ? 1 : 0;
-  if (use_graph && (e->graph_steps != chunk || e->graph_flags != fuse_baryon)) {
-    drop_graphs(e);
-    e->graph_steps = chunk;
-    e->graph_flags = fuse_baryon;
-  }
-  int issued = 0;
+  // both parities' graphs are captured before the first launch, so that no
+  // capture happens between launches of a run
+  if (use_graph && max_steps >= chunk)
+    if (int rc = ensure_graphs(e, chunk, fuse_baryon)) return rc;
+  int issued = 0, graph_launches = 0;
   while (issued < max_steps) {
     // the launches of a chunk start from the parity the device will be at
     // if no step is skipped; skipped steps (done flag) stop the run anyway
     const int parity = (c0 + issued) & 1;
-    if (use_graph) {
-      if (!e->graph[parity]) {
-        if (int rc = build_graph(e, parity, chunk, fuse_baryon)) return rc;
-      }
+    const int remaining = max_steps - issued;
+    if (use_graph && remaining >= chunk) {
       HIP_TRY(hipGraphLaunch(e->graph[parity], e->stream));
       issued += chunk;
-    } else {
-      const int n = std::min(chunk, max_steps - issued);
+      ++graph_launches;
+    } else {  // a partial chunk: direct launches of exactly the remaining steps
+      const int n = std::min(chunk, remaining);
       for (int s = 0; s < n; ++s) {
         hipError_t r = enqueue_run_step(e, (parity + s) & 1, fuse_baryon);
         if (r != hipSuccess) return fail(SKX_EHIP, std::string("stage launch: ") + hipGetErrorString(r));
@@ -430,6 +444,8 @@ int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, 
     out->err_abs = h->err_abs;
     out->baryon = h->baryon;
     out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    out->issued = issued;
+    out->graph_launches = graph_launches;
   }
   if (h->nonfinite) return fail(SKX_ENONFINITE, "non-finite energy or gradient during the run");
   return SKX_OK;
@@ -541,6 +557,66 @@ void release(skx_engine* e) {
   delete e;
 }
 
+bool observable_needs_com(int32_t kind) {
+  return kind == SKX_OBS_RMS || (kind >= SKX_OBS_QUADRUPOLE && kind != SKX_OBS_U);
+}
+
+// Argument checks of the observable entry points.
+int check_observable(const skx_engine* h, int32_t kind, const double* com, const double* out, int32_t n) {
+  static const int kCount[] = {4, 1, 9, 9, 9, 9, 1};
+  if (kind < SKX_OBS_COM || kind > SKX_OBS_D) return fail(SKX_EINVAL, "unknown observable kind");
+  if (kind >= SKX_OBS_QUADRUPOLE && h->L.theory != SKX_THEORY_PION)
+    return fail(SKX_EINVAL, "quadrupole, U, V, W and D exist for the nuclear theory only");
+  if (!out || n < kCount[kind]) return fail(SKX_EINVAL, "output holds fewer values than the observable has");
+  if (observable_needs_com(kind) && !com) return fail(SKX_EINVAL, "null centre of mass");
+  return SKX_OK;
+}
+
+// One observable pass: per-block partials into `partial` (device, `cap` bytes),
+// copied to `host` (at least as large) and folded there in block order with
+// Neumaier compensation, so the result is deterministic.
+int observable_pass(skx_engine* h, int32_t kind, const double* com, double* out, double* partial, size_t cap,
+                    double* host) {
+  static const int kCount[] = {4, 1, 9, 9, 9, 9, 1};
+  const int nv = kCount[kind];
+  const bool needs_com = observable_needs_com(kind);
+  const Lattice& L = h->L;
+  const int blocks = observable_blocks(L.N);
+  const size_t need = (size_t)blocks * nv * sizeof(double);
+  if (need > cap)
+    return fail(SKX_EINVAL, "observable pass needs " + std::to_string(need) + " bytes of partials, more than the " +
+                                std::to_string(cap) + " available");
+  ObsArgs a{};
+  a.kind = kind;
+  a.nv = nv;
+  a.X = L.X;
+  a.Y = L.Y;
+  a.Z = L.Z;
+  a.halo = L.halo;
+  a.lsx = L.lsx;
+  a.lsy = L.lsy;
+  a.lsz = L.lsz;
+  for (int i = 0; i < 3; ++i) a.com[i] = needs_com ? com[i] : 0.0;
+  a.P = L.P;
+  hipError_t r = L.prec == SKX_FP64
+                     ? launch_observable_f64(static_cast<const double*>(h->F[h->cur]), partial, a, blocks, h->stream)
+                     : launch_observable_f32(static_cast<const float*>(h->F[h->cur]), partial, a, blocks, h->stream);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("observable pass: ") + hipGetErrorString(r));
+  HIP_TRY(hipMemcpyAsync(host, partial, need, hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int q = 0; q < nv; ++q) {
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+      const double v = host[(size_t)b * nv + q];
+      const double t = s + v;
+      c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    out[q] = s + c;
+  }
+  return SKX_OK;
+}
+
 #define CHECK_HANDLE(h)                                                        \
   do {                                                                         \
     if (!(h)) return fail(SKX_EINVAL, "null handle");                          \
@@ -549,9 +625,12 @@ void release(skx_engine* e) {
 
 // Build a handle.  `shared` / `ext_st` / `ext_st_host` are non-null for a
 // lattice of a batch (shared stream, status in the batch's arrays);
-// `xchunk_all` > 0 forces one x-chunk for every stage.
+// `xchunk_all` > 0 forces one x-chunk for every stage.  `per_stage`: a tile
+// per stage (a solo handle asks for it with tile_z 0; a batch lattice takes
+// its batch's choice, so its own energy passes partition the lattice as a
+// solo handle's do).
 int create_engine(const skx_desc* desc, hipStream_t shared, DevStatus* ext_st, DevStatus* ext_st_host,
-                  int xchunk_all, skx_engine** out) {
+                  int xchunk_all, bool per_stage, skx_engine** out) {
   *out = nullptr;
   Lattice L;
   if (int rc = parse_desc(desc, L)) return rc;
@@ -577,7 +656,7 @@ int create_engine(const skx_desc* desc, hipStream_t shared, DevStatus* ext_st, D
     return fail(SKX_EHIP, std::string("hipDeviceGetAttribute: ") + hipGetErrorString(r));
   }
   for (int stg = 0; stg < 5; ++stg) {
-    const LaunchShape st = stage_tile(L, desc, tile, stg, shared == nullptr);
+    const LaunchShape st = stage_tile(L, tile, stg, per_stage);
     int bpc = 0;
     if ((r = stage_blocks_per_cu(L.prec == SKX_FP32, L.theory, stg, st, &bpc)) != hipSuccess) {
       delete e;
@@ -662,7 +741,7 @@ int skx_footprint_bytes(const skx_desc* desc, size_t* bytes) {
 
 int skx_create(const skx_desc* desc, skx_handle* out) {
   if (!out) return fail(SKX_EINVAL, "null output handle");
-  return create_engine(desc, nullptr, nullptr, nullptr, 0, out);
+  return create_engine(desc, nullptr, nullptr, nullptr, 0, desc->tile_z == 0, out);
 }
 
 int skx_destroy(skx_handle h) {
@@ -722,6 +801,13 @@ int skx_anf_run(skx_handle h, int32_t max_steps, double tol, int32_t check_every
   if (max_steps < 0) return fail(SKX_EINVAL, "max_steps must be >= 0");
   if (check_every < 0) return fail(SKX_EINVAL, "check_every must be >= 0");
   return run_loop(h, max_steps, tol, check_every, h->L.killkinen, stats);
+}
+
+int skx_anf_prepare(skx_handle h, int32_t check_every) {
+  CHECK_HANDLE(h);
+  if (check_every < 0) return fail(SKX_EINVAL, "check_every must be >= 0");
+  if (!graphs_enabled(h)) return SKX_OK;
+  return ensure_graphs(h, check_every > 0 ? check_every : 32, h->hist ? 1 : 0);
 }
 
 int skx_rk4_steps(skx_handle h, int32_t n, skx_run_stats* stats) {
@@ -792,53 +878,15 @@ int skx_init_field(skx_handle h, const skx_init_desc* desc) {
 
 int skx_observable(skx_handle h, int32_t kind, const double* com, double* out, int32_t n) {
   CHECK_HANDLE(h);
-  static const int kCount[] = {4, 1, 9, 9, 9, 9, 1};
-  if (kind < SKX_OBS_COM || kind > SKX_OBS_D) return fail(SKX_EINVAL, "unknown observable kind");
-  if (kind >= SKX_OBS_QUADRUPOLE && h->L.theory != SKX_THEORY_PION)
-    return fail(SKX_EINVAL, "quadrupole, U, V, W and D exist for the nuclear theory only");
-  const int nv = kCount[kind];
-  if (!out || n < nv) return fail(SKX_EINVAL, "output holds fewer values than the observable has");
-  const bool needs_com = kind == SKX_OBS_RMS || (kind >= SKX_OBS_QUADRUPOLE && kind != SKX_OBS_U);
-  if (needs_com && !com) return fail(SKX_EINVAL, "null centre of mass");
+  if (int rc = check_observable(h, kind, com, out, n)) return rc;
   if (!h->obs) {
     if (hipMalloc(&h->obs, (size_t)kObsMaxBlocks * 9 * sizeof(double)) != hipSuccess) {
       h->obs = nullptr;
       return fail(SKX_ENOMEM, "observable partials allocation failed");
     }
   }
-  const Lattice& L = h->L;
-  ObsArgs a{};
-  a.kind = kind;
-  a.nv = nv;
-  a.X = L.X;
-  a.Y = L.Y;
-  a.Z = L.Z;
-  a.halo = L.halo;
-  a.lsx = L.lsx;
-  a.lsy = L.lsy;
-  a.lsz = L.lsz;
-  for (int i = 0; i < 3; ++i) a.com[i] = needs_com ? com[i] : 0.0;
-  a.P = L.P;
-  const int blocks = observable_blocks(L.N);
-  hipError_t r = L.prec == SKX_FP64
-                     ? launch_observable_f64(static_cast<const double*>(h->F[h->cur]), h->obs, a, blocks, h->stream)
-                     : launch_observable_f32(static_cast<const float*>(h->F[h->cur]), h->obs, a, blocks, h->stream);
-  if (r != hipSuccess) return fail(SKX_EHIP, std::string("observable pass: ") + hipGetErrorString(r));
-  std::vector<double> part((size_t)blocks * nv);
-  HIP_TRY(hipMemcpyAsync(part.data(), h->obs, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  // block partials folded in block order with Neumaier compensation: deterministic
-  for (int q = 0; q < nv; ++q) {
-    double s = 0.0, c = 0.0;
-    for (int b = 0; b < blocks; ++b) {
-      const double v = part[(size_t)b * nv + q];
-      const double t = s + v;
-      c += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
-      s = t;
-    }
-    out[q] = s + c;
-  }
-  return SKX_OK;
+  std::vector<double> part((size_t)kObsMaxBlocks * 9);
+  return observable_pass(h, kind, com, out, h->obs, (size_t)kObsMaxBlocks * 9 * sizeof(double), part.data());
 }
 
 int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n) {
@@ -1169,7 +1217,7 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   skx_desc d = *lattice;
   d.tile_z = b->tile.tz;
   skx_engine* e = nullptr;
-  if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, xc, &e)) return rc;
+  if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, xc, b->nmaps > 1, &e)) return rc;
   const size_t n = (size_t)L.N * L.nf;
   if (F0) {
     if (int rc = copy_in(e, e->F[0], F0, n)) {
