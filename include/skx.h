@@ -188,6 +188,18 @@ enum skx_observable_kind {
 };
 int skx_observable(skx_handle h, int32_t kind, const double* com, double* out, int32_t n);
 
+/* skx_observable without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture set-up, tight serving loops, harnesses that forbid
+ * hipMalloc).  skx_observable allocates its block partials on a handle's
+ * first observable call; this variant keeps them in the library's static
+ * per-device array instead (2048 blocks x 9 doubles = 147456 bytes on each
+ * device, held until the process ends; a pass needing more is refused with
+ * SKX_EINVAL).  Same kernel, same fold order: results equal skx_observable's
+ * bit for bit.  Every call on one device shares that array, so calls that
+ * overlap on one device (from several threads) race; serialise them. */
+int skx_observable_noalloc(skx_handle h, int32_t kind, const double* com, double* out, int32_t n);
+
 /* ---------------------------------------------------------------- initial data
  * Device-side initial configurations.  Replaces the host-built field of
  * Simulation.initialize -> theory initialize (core/simulation.py:124-173) with

@@ -39,12 +39,13 @@ OBS_COUNT = (4, 1, 9, 9, 9, 9, 1)
 
 EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
-    "skx_anf_run", "skx_anf_prepare", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density", "skx_eval_gradient",
+    "skx_anf_run", "skx_anf_prepare", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density",
+    "skx_eval_gradient",
     "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_sync",
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
     "skx_batch_remove", "skx_diag_stream_copy", "skx_device_info", "skx_last_error", "skx_version",
-    "skx_init_field", "skx_batch_init", "skx_observable", "skx_write_dat",
+    "skx_init_field", "skx_batch_init", "skx_observable", "skx_observable_noalloc", "skx_write_dat",
 )
 
 
@@ -135,6 +136,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_baryon": ([H, dp], C.c_int),
         "skx_density": ([H, C.c_int32, dp, C.c_size_t], C.c_int),
         "skx_observable": ([H, C.c_int32, dp, dp, C.c_int32], C.c_int),
+        "skx_observable_noalloc": ([H, C.c_int32, dp, dp, C.c_int32], C.c_int),
         "skx_write_dat": ([C.c_char_p, dp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32],
                           C.c_int),
         "skx_eval_gradient": ([H, dp, dp, dp, C.c_size_t], C.c_int),

@@ -143,6 +143,9 @@ constexpr int kObsMaxBlocks = 2048;
 int observable_blocks(long long N);
 hipError_t launch_observable_f64(const double* F, double* partial, const ObsArgs& a, int blocks, hipStream_t st);
 hipError_t launch_observable_f32(const float* F, double* partial, const ObsArgs& a, int blocks, hipStream_t st);
+// Per-device address and size of the observables' static partials array (the
+// allocation-free entry point's scratch); `device` must be current.
+hipError_t observable_scratch(int device, double** ptr, size_t* bytes);
 
 hipError_t launch_fill(void* p, size_t bytes, hipStream_t st);  // zero fill kernel (graph safe)
 hipError_t launch_apply_arrest_f64(double* V, long long n, const DevStatus* st, hipStream_t s);

@@ -15,9 +15,18 @@
 //   D     1: (r_i r_j T_ij - r^2 tr T / 3) vol             (:977-1033)
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "skx_internal.h"
 
 namespace skx {
+
+// Block partials of the allocation-free observable entry point
+// (skx_observable_noalloc): kObsMaxBlocks blocks x 9 values, 147456 bytes on
+// every device the library runs on, held until the process ends.  Host code
+// never uses this name as an address: it takes the per-device address from
+// observable_scratch() below.
+static __device__ __align__(256) double scratch[kObsMaxBlocks * 9];
 
 namespace {
 
@@ -237,6 +246,30 @@ hipError_t launch_observable(const T* F, double* partial, const ObsArgs& a, int 
 
 hipError_t launch_observable_f64(const double* F, double* partial, const ObsArgs& a, int blocks, hipStream_t st) {
   return launch_observable<double>(F, partial, a, blocks, st);
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on `device`, which the caller has made current.
+// Looked up with hipGetSymbolAddress on the first call per device and kept:
+// each device holds its own copy of the array at its own address.
+hipError_t observable_scratch(int device, double** ptr, size_t* bytes) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (r != hipSuccess) return r;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = static_cast<double*>(p);
+  }
+  *ptr = g_scratch_ptr[device];
+  *bytes = sizeof(scratch);
+  return hipSuccess;
 }
 hipError_t launch_observable_f32(const float* F, double* partial, const ObsArgs& a, int blocks, hipStream_t st) {
   return launch_observable<float>(F, partial, a, blocks, st);

@@ -889,6 +889,18 @@ int skx_observable(skx_handle h, int32_t kind, const double* com, double* out, i
   return observable_pass(h, kind, com, out, h->obs, (size_t)kObsMaxBlocks * 9 * sizeof(double), part.data());
 }
 
+int skx_observable_noalloc(skx_handle h, int32_t kind, const double* com, double* out, int32_t n) {
+  CHECK_HANDLE(h);
+  if (int rc = check_observable(h, kind, com, out, n)) return rc;
+  double* partial = nullptr;
+  size_t bytes = 0;
+  hipError_t r = observable_scratch(h->device, &partial, &bytes);
+  if (r != hipSuccess) return fail(SKX_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+  // host copy of the partials: a fixed per-thread buffer of the scratch's size, no allocation per call
+  static thread_local double part[kObsMaxBlocks * 9];
+  return observable_pass(h, kind, com, out, partial, bytes, part);
+}
+
 int skx_eval_gradient(skx_handle h, const double* temp, double* eg, double* l, size_t n) {
   CHECK_HANDLE(h);
   if (!temp) return fail(SKX_EINVAL, "null stage field");
