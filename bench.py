@@ -65,6 +65,8 @@ def time_form(prec, p_i, p_f, F, steps, warmup, chunk, trace=True, theory=None, 
     out = {}
     eng = _engine.Engine(theory, prec, p_i, p_f)
     eng.upload(_engine.FIELD, F)
+    eng.prepare(chunk)
+    tr = None
     if trace:
         tr = _engine.Engine(theory, prec, p_i, p_f, flags=_engine.FLAG_TRACE)
         tr.upload(_engine.FIELD, F)
@@ -74,9 +76,7 @@ def time_form(prec, p_i, p_f, F, steps, warmup, chunk, trace=True, theory=None, 
         ms, cnt = tr.trace_read()
         out["kernels"] = {STAGE_NAMES[k]: dict(avg_us=1000.0 * ms[k] / cnt[k], launches=int(cnt[k]))
                           for k in range(5) if cnt[k]}
-        tr.close()
     eng.anf_run(warmup, -1.0, chunk)
-    eng.prepare(chunk)
     eng.sync()
     t0 = time.perf_counter()
     st = eng.anf_run(steps, -1.0, chunk)
@@ -87,6 +87,8 @@ def time_form(prec, p_i, p_f, F, steps, warmup, chunk, trace=True, theory=None, 
     out.update(seconds=dt, steps=st.steps, issued=st.issued, graph_launches=st.graph_launches, energy=st.energy,
                err_abs=st.err_abs, baryon=st.baryon)
     eng.close()
+    if tr is not None:
+        tr.close()
     return out
 
 
@@ -104,8 +106,9 @@ def profiled_traffic(kernel: str, form: str):
 
 
 def solve_latency(prec, p_i, p_f, seeds, tol, max_steps):
-    """Time-to-converged (max|g| < tol) from each seed's initial field, with the converged E and B."""
-    lat, steps, energy, baryon = [], [], [], []
+    """Time-to-converged (max|g| < tol) from each seed's initial field, with the converged E and B; also
+    returns the converged fields."""
+    lat, steps, energy, baryon, conv, fields = [], [], [], [], [], []
     for s in seeds:
         _, _, _, F = workload(seed=s)
         eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
@@ -118,26 +121,65 @@ def solve_latency(prec, p_i, p_f, seeds, tol, max_steps):
         steps.append(st.steps)
         energy.append(st.energy)
         baryon.append(st.baryon)
+        conv.append(bool(st.converged))
+        fields.append(eng.download(_engine.FIELD))
         eng.close()
     lat = np.array(lat)
     return dict(tol=tol, seeds=len(seeds), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                steps=steps, energy=energy, baryon=baryon)
+                steps=steps, converged=conv, energy=energy, baryon=baryon), fields
 
 
-def cpu_baseline(p_i, p_f, F, steps: int = 1):
-    """FP64 NumPy oracle (single-threaded) on the host: `steps` ANF steps of the same 128^3 workload."""
+def host_identity():
+    """Host CPU identity: `nproc`, the cores this process may use, the model name and the sockets
+    (from /proc/cpuinfo)."""
+    model, sockets = None, set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            key, _, val = line.partition(":")
+            key = key.strip()
+            if key == "model name" and model is None:
+                model = val.strip()
+            elif key == "physical id":
+                sockets.add(val.strip())
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(nproc=os.cpu_count(), usable_cores=usable, model=model, sockets=len(sockets) or None)
+
+
+def cpu_baseline(p_i, p_f, F, steps: int = 3, threads: int | None = None, threads_steps: int = 5):
+    """FP64 NumPy oracle of the same 128^3 workload on the host: `steps` ANF steps on one thread, and
+    `threads_steps` steps with the stencil, point-gradient and RK4 passes split over x-slabs on `threads`
+    threads (default: the usable cores, at most 16, the host share of one GPU).  `value` is the
+    all-threads figure; both runs take the same arithmetic (they agree bit for bit)."""
     from oracle import integrator as OI
+    from oracle import parallel as OP
 
+    N = int(p_i[6])
+    host = host_identity()
     st = OI.OracleState("pion", p_i, p_f, F)
     E = OI.energy(st)
     t0 = time.perf_counter()
     for _ in range(steps):
         E, _ = OI.anf_step(st, E)
-    dt = time.perf_counter() - t0
-    N = int(p_i[6])
-    return dict(value=N * steps / dt, unit="site-updates/s", cores=1, kind="port",
-                sample=f"{steps} ANF step(s) of the 128^3 B=4 pion workload, FP64 NumPy oracle, 1 thread, "
-                       f"{dt:.1f} s")
+    t1 = time.perf_counter() - t0
+    threads = threads or max(1, min(16, host["usable_cores"] or 1))
+    pool = OP.SlabPool(int(p_i[0]), threads)
+    try:
+        st = OI.OracleState("pion", p_i, p_f, F)
+        E = OP.energy(st, pool)
+        t0 = time.perf_counter()
+        for _ in range(threads_steps):
+            E, _ = OP.anf_step(st, E, pool)
+        tn = time.perf_counter() - t0
+    finally:
+        pool.close()
+    return dict(value=N * threads_steps / tn, unit="site-updates/s", cores=threads, kind="port",
+                sample=f"{threads_steps} ANF steps of the 128^3 B=4 pion workload, FP64 NumPy oracle split over "
+                       f"x-slabs on {threads} threads, {tn:.1f} s",
+                single_thread=dict(value=N * steps / t1, cores=1,
+                                   sample=f"{steps} ANF steps, same oracle on 1 thread, {t1:.1f} s"),
+                host=host)
 
 
 def batch_sample(n: int, steps: int):
@@ -179,24 +221,79 @@ def batch_sample(n: int, steps: int):
                 blocks=blocks)
 
 
-def serving_sample(n: int, rate: float, sizes, max_steps: int):
-    """Serving form: a Poisson stream of `n` minimisation requests at `rate` per second (pion 60 %, omega
-    25 %, rho 15 %; L from `sizes`; B = 1..8) served to max|g| < 1e-4 by continuous batching, in the FP32
-    and FP64 forms and in FP32 with each lattice finished in FP64 once max|g| < 3 tol; time-to-converged
-    p50 / p99 and throughput."""
+def c3_sample(steps: int = 2000, theories=("omega", "coulomb")):
+    """160^3 B=8 omega and Coulomb (product of two B=4 rational maps at -+1.5 along x, dx = 0.15; fields
+    built on the device): `steps` ANF steps in the FP64 and FP32 forms, each timed as one device-resident
+    run, and the FP32 result against the FP64 one (energy, baryon number, field rel-L2)."""
+    from skyrmion_solver_amd import workloads
+
+    tids = {"omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB}
+    out = {}
+    for th in theories:
+        _, rp, p_i, p_f, init = workloads.c3_b8(th, device_init=True)
+        N = int(p_i[6])
+        r = {}
+        for name, prec in (("fp64", _engine.FP64), ("fp32", _engine.FP32)):
+            with _engine.Engine(tids[th], prec, p_i, p_f) as eng:
+                eng.init_field(init)
+                B0 = eng.baryon()
+                eng.prepare(100)
+                t0 = time.perf_counter()
+                st = eng.anf_run(steps, -1.0, 100)
+                eng.sync()
+                dt = time.perf_counter() - t0
+                r[name] = dict(site_updates_per_s=N * st.steps / dt, ms_per_step=1000.0 * dt / st.steps,
+                               steps=st.steps, energy=st.energy, baryon=st.baryon, baryon_initial=B0,
+                               err_abs=st.err_abs)
+                r[name + "_field"] = eng.download(_engine.FIELD)
+        F64, F32 = r.pop("fp64_field"), r.pop("fp32_field")
+        r["fp32_vs_fp64"] = dict(abs_dE=abs(r["fp32"]["energy"] - r["fp64"]["energy"]),
+                                 rel_dE=abs(r["fp32"]["energy"] - r["fp64"]["energy"]) / abs(r["fp64"]["energy"]),
+                                 abs_dB=abs(r["fp32"]["baryon"] - r["fp64"]["baryon"]),
+                                 field_rel_l2=float(np.linalg.norm(F32 - F64) / np.linalg.norm(F64)))
+        r["fp32_over_fp64"] = r["fp32"]["site_updates_per_s"] / r["fp64"]["site_updates_per_s"]
+        r["algorithmic_GBs_fp32"] = r["fp32"]["site_updates_per_s"] * 16 * 5 * 4 / 1e9
+        out[th] = r
+    return out
+
+
+def _serve_summary(rep: dict) -> dict:
+    """Time-to-result over all requests and time-to-converged over the converged ones, with the counts
+    of requests that converged, stopped at their step limit and went non-finite."""
+    by = rep["by_theory"]
+    nonfinite = sum(t["nonfinite"] for t in by.values())
+    return dict(requests=rep["requests"], converged=rep["converged"],
+                stopped_at_step_limit=rep["requests"] - rep["converged"] - nonfinite, nonfinite=nonfinite,
+                converged_p50_s=rep["converged_p50_s"], converged_p99_s=rep["converged_p99_s"],
+                p50_s=rep["p50_s"], p99_s=rep["p99_s"], makespan_s=rep["makespan_s"],
+                site_updates_per_s=rep["site_updates_per_s"], refined=rep["refined"], by_theory=by,
+                peak_hbm_bytes=rep["peak_hbm_bytes"])
+
+
+def serving_sample(n: int, rates, sizes, max_steps: int, compare_rate: float):
+    """Serving form: Poisson streams of `n` minimisation requests (pion 60 %, omega 25 %, rho 15 %; L from
+    `sizes`; B = 1..8; step limit `max_steps`, four times that for the slowly relaxing omega flow) served
+    to max|g| < 1e-4 by continuous batching.  At every rate of `rates` the FP32 form with each lattice
+    finished in FP64 once max|g| < 3 tol; at `compare_rate` also the plain FP32 and FP64 forms."""
     from skyrmion_solver_amd import serving, workloads
 
-    out = {}
-    for name, prec, refine in (("fp32", _engine.FP32, False), ("fp64", _engine.FP64, False),
-                               ("fp32_refined_fp64", _engine.FP32, True)):
-        reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps, device_init=True)
-        srv = serving.Server(precision=prec, hbm_budget_bytes=64 << 30, chunk_steps=32, max_slots=64,
-                             max_blocks=32768, refine_fp64=refine)
-        rep = srv.serve(reqs)
-        srv.close()
-        out[name] = rep
-    out.update(requests=n, rate_per_s=rate, sizes=list(sizes), max_steps=max_steps)
-    return out
+    runs = []
+    for rate in rates:
+        forms = [("fp32_refined_fp64", _engine.FP32, True)]
+        if rate == compare_rate:
+            forms = [("fp32", _engine.FP32, False), ("fp64", _engine.FP64, False)] + forms
+        for name, prec, refine in forms:
+            reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps, device_init=True)
+            srv = serving.Server(precision=prec, hbm_budget_bytes=64 << 30, chunk_steps=32, max_slots=64,
+                                 max_blocks=32768, refine_fp64=refine)
+            rep = srv.serve(reqs)
+            srv.close()
+            runs.append(dict(form=name, rate_per_s=rate, **_serve_summary(rep)))
+            print(f"[bench] serving {name} rate {rate}: converged {rep['converged']}/{rep['requests']}, "
+                  f"converged p50 {rep['converged_p50_s']} s, makespan {rep['makespan_s']:.1f} s",
+                  file=sys.stderr, flush=True)
+    return dict(requests_per_stream=n, sizes=list(sizes), max_steps=max_steps,
+                step_budget_factor=dict(workloads.STEP_BUDGET_FACTOR), tol=1.0e-4, runs=runs)
 
 
 def main():
@@ -214,10 +311,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-n", type=int, default=48, help="lattices of the batched sample (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=40)
-    ap.add_argument("--serve-n", type=int, default=24, help="requests of the serving sample (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-sizes", type=int, nargs="+", default=[64, 96])
-    ap.add_argument("--serve-max-steps", type=int, default=8000)
+    ap.add_argument("--serve-n", type=int, default=8, help="requests per serving stream (0: skip)")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[5.0, 20.0, 200.0])
+    ap.add_argument("--serve-compare-rate", type=float, default=20.0,
+                    help="rate at which the plain FP32 and FP64 forms are served too")
+    ap.add_argument("--serve-sizes", type=int, nargs="+", default=[64, 96, 128, 192])
+    ap.add_argument("--serve-max-steps", type=int, default=2500,
+                    help="step limit of a serving request (omega requests: four times this)")
+    ap.add_argument("--c3-steps", type=int, default=2000, help="ANF steps of the 160^3 B=8 omega / Coulomb sample "
+                                                               "(0: skip)")
     ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
                     help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
     args = ap.parse_args()
@@ -263,28 +365,33 @@ def main():
                         bytes_per_launch=bytes_per_launch, measured_copy_GBs=copy_gbs,
                         frac_of_copy=achieved / copy_gbs)
 
-    latency = {}
+    latency, fields = {}, {}
     if args.solve_seeds > 0:
         for name, prec in (("fp32", _engine.FP32), ("fp64", _engine.FP64)):
-            latency[name] = solve_latency(prec, p_i, p_f, list(range(args.solve_seeds)), args.tol,
-                                          args.solve_max_steps)
+            latency[name], fields[name] = solve_latency(prec, p_i, p_f, list(range(args.solve_seeds)), args.tol,
+                                                        args.solve_max_steps)
             print(f"[bench] solve {name}: {latency[name]}", file=sys.stderr, flush=True)
 
     accuracy = None
     if "fp32" in latency and "fp64" in latency:
         e32, e64 = np.array(latency["fp32"]["energy"]), np.array(latency["fp64"]["energy"])
         b32, b64 = np.array(latency["fp32"]["baryon"]), np.array(latency["fp64"]["baryon"])
+        l2 = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fields["fp32"], fields["fp64"])]
         accuracy = dict(max_rel_dE=float(np.max(np.abs(e32 - e64) / np.abs(e64))),
-                        max_abs_dB=float(np.max(np.abs(b32 - b64))),
-                        note="FP32 form vs FP64 form at convergence (max|g| < tol), same seeds")
+                        max_abs_dB=float(np.max(np.abs(b32 - b64))), max_field_rel_l2=max(l2),
+                        converged=all(latency["fp32"]["converged"]) and all(latency["fp64"]["converged"]),
+                        note="FP32 form vs FP64 form at convergence (max|g| < tol), same seeds; the FP64 form "
+                             "matches the FP64 oracle step for step at this size")
+    fields.clear()
+    c3 = c3_sample(args.c3_steps) if args.c3_steps > 0 else None
+    if c3:
+        print(f"[bench] c3: {c3}", file=sys.stderr, flush=True)
 
     batch = batch_sample(args.batch_n, args.batch_steps) if args.batch_n > 0 else None
     if batch:
         print(f"[bench] batch: {batch}", file=sys.stderr, flush=True)
-    serve = serving_sample(args.serve_n, args.serve_rate, args.serve_sizes, args.serve_max_steps) \
-        if args.serve_n > 0 else None
-    if serve:
-        print(f"[bench] serving: {serve}", file=sys.stderr, flush=True)
+    serve = serving_sample(args.serve_n, args.serve_rates, args.serve_sizes, args.serve_max_steps,
+                           args.serve_compare_rate) if args.serve_n > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -313,6 +420,7 @@ def main():
         "fp32_over_fp64": head["site_updates_per_s"] / res["fp64"]["site_updates_per_s"],
         "solve_latency": latency,
         "fp32_accuracy": accuracy,
+        "c3_160_b8": c3,
         "batch": batch,
         "serving": serve,
         "roofline": roofline,

@@ -100,17 +100,20 @@ __device__ cplx rational_map(cplx z, int b) {
   return p;
 }
 
-__device__ double fit_profile(const double (*c)[5], double r, int b) {
-  if (b < 1 || b > 9) return kPi;  // all-zero coefficients
+// pi (1 - tanh |r (a + r (b + ...))|) of the fit for B = b; `outside` for b beyond 1..9
+__device__ double fit_profile(const double (*c)[5], double r, int b, double outside) {
+  if (b < 1 || b > 9) return outside;
   const double* k = c[b - 1];
   const double poly = r * (k[0] + r * (k[1] + r * (k[2] + r * (k[3] + r * k[4]))));
   return kPi * (1.0 - tanh(fabs(poly)));
 }
 
-// profilefun (initial_config.py:343-442)
+// profilefun (initial_config.py:343-442).  Beyond B = 9 the reference's
+// coefficients stay 0: the massive fit is then pi (1 - tanh 0) = pi and the
+// massless term of the m0 < 1 branch is never evaluated (it stays 0).
 __device__ double profilefun(double r, int b, double m0) {
-  double f = (m0 < 1.0) ? (1.0 - m0) * fit_profile(kFitMassless, r, b) + m0 * fit_profile(kFitMassive, r, b)
-                        : fit_profile(kFitMassive, r, b);
+  double f = (m0 < 1.0) ? (1.0 - m0) * fit_profile(kFitMassless, r, b, 0.0) + m0 * fit_profile(kFitMassive, r, b, kPi)
+                        : fit_profile(kFitMassive, r, b, kPi);
   if (r > 15.0) f = 0.0;
   if (r < 1.0e-6) f = kPi;
   return f;

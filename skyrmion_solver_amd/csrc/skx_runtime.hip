@@ -420,10 +420,15 @@ int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, 
       }
       issued += n;
     }
+    // Without a tolerance (tol < 0) a run stops early only on a non-finite
+    // value: after its last chunk the final energy pass is enqueued at once,
+    // on the parity of all steps taken, and checked after the one read below.
+    if (issued >= max_steps && !(tol >= 0.0)) break;
     if (int rc = pull_status(e)) return rc;
     if (h->done) break;
   }
-  const int steps = h->steps;
+  const bool speculative = issued >= max_steps && !(tol >= 0.0);
+  int steps = speculative ? issued : h->steps;
   e->cur = (c0 + steps) & 1;
   // energy of the final field, the last step's arrest decision, and its application
   {
@@ -435,6 +440,14 @@ int run_loop(skx_engine* e, int32_t max_steps, double tol, int32_t check_every, 
     if (r != hipSuccess) return fail(SKX_EHIP, std::string("arrest: ") + hipGetErrorString(r));
   }
   if (int rc = pull_status(e)) return rc;
+  if (h->steps != steps) {
+    // a non-finite value stopped the run early: report the field it stopped at
+    steps = h->steps;
+    e->cur = (c0 + steps) & 1;
+    hipError_t r = launch(e, 0, FUSE_ENERGY | FUSE_BARYON, e->cur);
+    if (r != hipSuccess) return fail(SKX_EHIP, std::string("energy pass: ") + hipGetErrorString(r));
+    if (int rc = pull_status(e)) return rc;
+  }
   auto t1 = std::chrono::steady_clock::now();
   if (out) {
     out->steps = steps;

@@ -92,7 +92,8 @@ def profilefun(r, bint: int, m0: float = 1.0):
     """Fitted radial profile (``initial_config.py:343-442``)."""
     r = np.asarray(r, dtype=np.float64)
     if m0 < 1.0:
-        f = _ffun(r, _PROFILE_FITS_MASSLESS.get(bint, (0, 0, 0, 0, 0)))
+        # beyond B = 9 the reference leaves the massless term at 0 (and the massive coefficients at 0)
+        f = _ffun(r, _PROFILE_FITS_MASSLESS[bint]) if bint in _PROFILE_FITS_MASSLESS else np.zeros_like(r)
         fm = _ffun(r, _PROFILE_FITS.get(bint, (0, 0, 0, 0, 0)))
         out = (1.0 - m0) * f + m0 * fm
     else:

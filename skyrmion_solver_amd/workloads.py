@@ -25,6 +25,9 @@ COURANT_BY_THEORY = {"pion": 0.2, "omega": 0.2, "coulomb": 0.2, "rho": 0.05, "be
 THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model", "coulomb": "Coulomb-Skyrme model",
                 "rho": "Rho-Skyrme model", "berger": "Berger Skyrme model", "lightly": "Lightly bound Skyrme model"}
 SERVE_MIX = (("pion", 0.60), ("omega", 0.25), ("rho", 0.15))
+# step budget of a serving request per theory, as a multiple of the stream's max_steps: the omega saddle
+# flow (pion field descending while omega ascends) relaxes in 10k-38k steps at 64^3, B = 1..4
+STEP_BUDGET_FACTOR = {"pion": 1, "omega": 4, "rho": 1}
 
 
 def _lattice(L, dx, courant=COURANT):
@@ -53,23 +56,34 @@ def c2_b4(seed: int = 0, L: int = 128):
     return th, rp, p_i, p_f, F
 
 
-def c3_b8(theory: str = "omega", seed: int = 0, L: int = 160):
+def c3_b8(theory: str = "omega", seed: int = 0, L: int = 160, device_init: bool = False):
     """Omega or Coulomb Skyrme, B=8 = product of two B=4 rational maps at -+1.5 along x, 160^3, dx = 0.15.
 
     omega: c_omega = 0.5 (omega mass 1 in kernel units), pion mass the
     derived default m_pi / m_omega; omega starts at -c_omega * CS.
     Coulomb: kappa = 0.1, pion mass the derived default; V starts at 0.
+    With `device_init` the last entry is the descriptor that builds the field on the device (omega
+    included) instead of the host field (whose omega the caller sets, ``set_omega_from_chern_simons``).
     """
     name = {"omega": "Omega-Skyrme model", "coulomb": "Coulomb-Skyrme model"}[theory]
     th = load_theory(name)
     extra = dict(c_omega=0.5) if theory == "omega" else dict(kappa=0.1)
     rp = th.params.default_params(**_lattice(L, 0.15), baryon_number=8.0, **extra).resolved()
     p_i, p_f = th.params.pack_device_params(rp)
-    lump = dict(mode="rational_map", baryon_number=4, profile=("exp", 1.5))
-    F = initial_data.build_field(rp, 5, dict(mode="product", seed=seed, vacuum_band=True,
-                                            lumps=[dict(lump, center=(-1.5, 0.0, 0.0)),
-                                                   dict(lump, center=(1.5, 0.0, 0.0))]))
+    cfg = c3_b8_config(seed)
+    if device_init:
+        c_cs = float(rp.c_omega) if theory == "omega" else 0.0
+        return th, rp, p_i, p_f, initial_data.device_init_desc(rp, cfg, omega_cs=c_cs)
+    F = initial_data.build_field(rp, 5, cfg)
     return th, rp, p_i, p_f, F
+
+
+def c3_b8_config(seed: int = 0) -> dict:
+    """Initial-data request of the 160^3 B=8 workload: the product of two B=4 rational maps
+    f = pi e^{-r/1.5} centred at -+1.5 along x, vacuum band."""
+    lump = dict(mode="rational_map", baryon_number=4, profile=("exp", 1.5))
+    return dict(mode="product", seed=seed, vacuum_band=True,
+                lumps=[dict(lump, center=(-1.5, 0.0, 0.0)), dict(lump, center=(1.5, 0.0, 0.0))])
 
 
 def c4_instances(n: int = 48, seed: int = 0, sizes=(48, 64, 96, 128), dx: float = 0.15, device_init: bool = False):
@@ -165,7 +179,8 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
 
 def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 128, 160, 192), mix=SERVE_MIX,
                 tol: float = 1.0e-4, max_steps: int = 20000, device_init: bool = False):
-    """Poisson stream of `n` requests at `rate` per second: theory drawn from `mix`, L from `sizes`, B = 1..8."""
+    """Poisson stream of `n` requests at `rate` per second: theory drawn from `mix`, L from `sizes`, B = 1..8;
+    each request's step limit is `max_steps` times its theory's ``STEP_BUDGET_FACTOR``."""
     from skyrmion_solver_amd.serving import poisson_arrivals
 
     rng = np.random.default_rng(seed)
@@ -178,6 +193,7 @@ def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 
         theory = str(rng.choice(names, p=probs))
         L = int(rng.choice(sizes))
         B = int(rng.integers(1, 9))
-        out.append(serving_request(k, theory, L, B, rng, tol=tol, max_steps=max_steps, arrival=float(arrivals[k]),
-                                   device_init=device_init))
+        out.append(serving_request(k, theory, L, B, rng, tol=tol,
+                                   max_steps=int(max_steps * STEP_BUDGET_FACTOR.get(theory, 1)),
+                                   arrival=float(arrivals[k]), device_init=device_init))
     return out

@@ -4,6 +4,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from oracle import initial_config as OIC
 from skyrmion_solver_amd import initial_data
 from skyrmion_solver_amd.theories import load_theory
 
@@ -59,6 +60,72 @@ def smooth_field(rp, nf: int, seed: int = 0, noise: float = 0.02, normalise: boo
     if band:
         initial_data.set_vacuum_band(F)
     return F
+
+
+def oracle_field(rp, p_i, p_f, nf, cfg):
+    """The oracle's field for a build request: the lumps ``initial_data.lump_specs`` draws (centres,
+    rotation matrices, profiles), each evaluated by the oracle, multiplied in order, then the band."""
+    lumps, extra = initial_data.lump_specs(rp, cfg)
+    acc = None
+    for sp in lumps:
+        prof = sp["profile"]
+        if prof is None:
+            if sp["mode"] == "hedgehog":
+                prof = ("gauss", max(max(p_f[0], p_f[1], p_f[2]) / (8.0 * max(abs(sp["B"]), 1.0)), 1.0e-6))
+            else:
+                prof = ("fit", 1.0)
+        elif prof == "fit":
+            prof = ("fit", 1.0)
+        if sp["rotation"] is not None:
+            iso = sp["rotation"]
+        else:
+            iso = OIC.euler_matrix(sp["iso"], 0.0, 0.0) if sp["mode"] == "hedgehog" else OIC.euler_matrix(0.0, 0.0,
+                                                                                                    sp["iso"])
+        part = OIC.lump(p_i, p_f, sp["mode"], sp["B"], tuple(prof), sp["center"], sp["space_rotation"], iso)
+        acc = part if acc is None else OIC.product_ansatz(acc, part)
+    F = np.zeros((nf, int(p_i[0]), int(p_i[1]), int(p_i[2])))
+    F[0] = 1.0
+    if acc is not None:
+        F[:4] = acc
+    if extra["vacuum_band"]:
+        OIC.vacuum_band(F, int(p_i[3]))
+    return F
+
+
+def ill_conditioned_sites(rp, p_i, p_f, cfg, shift: float = 1.0e-11, tol: float = 1.0e-7, dilate: int = 0):
+    """Sites where a request's ansatz is ill-conditioned, as a boolean (X, Y, Z) mask.
+
+    Where a site lies exactly on a direction in which a rational map's denominator vanishes (e.g. four
+    of the cube diagonals of the B = 4 map, hit by lattices whose lump centres sit half a spacing off
+    the sites), the reference's regularised division (1e-18 added to |den|^2) returns a value set by
+    rounding noise, so implementations with different rounding (device math library, operand order)
+    differ there.  Detected per rational-map lump as the sites whose map direction n(R(z)) moves by
+    more than `tol` when the lump centre moves by `shift` in a generic direction (the profile is left
+    out: far from a lump sin f is small and would hide the jump); `dilate` widens the mask by that many
+    sites along each axis (for quantities built from stencils over the field)."""
+    lumps, _ = initial_data.lump_specs(rp, cfg)
+    rx, ry, rz = OIC.centered_position(p_i, p_f)
+    step = shift * np.array([1.0, 0.37, -0.61])  # off every symmetry axis and diagonal
+    mask = np.zeros(rx.shape, dtype=bool)
+    for sp in lumps:
+        if sp["mode"] != "rational_map":
+            continue
+        S = np.eye(3) if sp["space_rotation"] is None else np.asarray(sp["space_rotation"])
+        bint = max(int(abs(sp["B"]) + 0.5), 1)
+
+        def direction(c):
+            x, y, z = OIC.apply_matrix(S, rx - c[0], ry - c[1], rz - c[2])
+            z0, z1, _ = OIC.stereographic(x, y, z)
+            return np.array(OIC.zton(*OIC.rational_map_value(z0, z1, bint)))
+
+        c = np.asarray(sp["center"], dtype=np.float64)
+        mask |= np.max(np.abs(direction(c + step) - direction(c)), axis=0) > tol
+    for _ in range(dilate):
+        grown = mask.copy()
+        for ax in range(3):
+            grown |= np.roll(mask, 1, axis=ax) | np.roll(mask, -1, axis=ax)
+        mask = grown
+    return mask
 
 
 def rel_l2(a, b) -> float:

@@ -1,8 +1,13 @@
-"""GPU: device-side initial data equals the host builder (initial_data.build_field) on the same request.
+"""GPU: device-side initial data equals the oracle's restatement of the
+reference's initial configurations (oracle/initial_config.py, pinned by
+tests/test_oracle_initial_config.py) on the same request.
 
-Transcendentals (cos, sin, exp, acos, atan2, tan, tanh) come from the device
-math library, so the FP64 device field matches the host field to 1e-12
-absolute rather than bitwise; the FP32 form to FP32 rounding.
+The request's random draws (centres, rotation matrices) come from
+initial_data.lump_specs; the oracle evaluates the lumps, their product and the
+vacuum band on its own.  Transcendentals (cos, sin, exp, acos, atan2, tan,
+tanh) come from the device math library, so the FP64 device field matches
+the oracle to 1e-12 absolute rather than bitwise; the FP32 form to FP32
+rounding.
 """
 
 from __future__ import annotations
@@ -10,7 +15,8 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import make_params
+from helpers import make_params, oracle_field
+from oracle import initial_config as OIC
 from skyrmion_solver_amd import _engine, initial_data
 
 pytestmark = pytest.mark.gpu
@@ -24,6 +30,9 @@ CONFIGS = [
                 dict(mode="hedgehog", center=(0.9, 0.1, 0.0), profile=("gauss", 0.7), isorotation="haar")]),
 ] + [dict(mode="rational_map", baryon_number=B, profile=("exp", 1.2), isorotation="haar", space_rotation="haar",
           seed=B, isospin_rotation=0.3) for B in range(1, 10)] + [
+    # the reference's fitted profiles (massive, and mixed with the massless fit), every B, and beyond the fits
+    dict(mode="rational_map", baryon_number=B, profile=("fit", m0), isospin_rotation=0.2 * B)
+    for B in range(1, 11) for m0 in (1.0, 0.35)] + [
     dict(mode="smorgasbord", baryon_number=5, seed=11, vacuum_band=True),
     dict(mode="smorgasbord", count=16, lump_baryon=1, profile_m0=0.6, position_scale=0.25, seed=3),
     dict(mode="smorgasbord", count=3, lump_baryon=2, seed=7),
@@ -37,13 +46,13 @@ def _device_field(theory, prec, rp, p_i, p_f, cfg, omega_cs=0.0):
 
 
 @pytest.mark.parametrize("k", range(len(CONFIGS)))
-def test_device_init_matches_host_fp64(k):
+def test_device_init_matches_oracle_fp64(k):
     cfg = CONFIGS[k]
     print(f"[case] device init {cfg}", flush=True)
     mod, rp, p_i, p_f = make_params("pion", 24, 22, 20)
-    F_host = initial_data.build_field(rp, 4, cfg).ravel()
+    F_ref = oracle_field(rp, p_i, p_f, 4, cfg).ravel()
     F, V = _device_field("pion", _engine.FP64, rp, p_i, p_f, cfg)
-    assert np.max(np.abs(F - F_host)) <= 1e-12
+    assert np.max(np.abs(F - F_ref)) <= 1e-12
     assert np.max(np.abs(V)) == 0.0
 
 
@@ -51,9 +60,9 @@ def test_device_init_fp32_and_extra_components():
     cfg = dict(mode="rational_map", baryon_number=3, profile=("exp", 1.1), isorotation="haar", seed=7,
                vacuum_band=True)
     mod, rp, p_i, p_f = make_params("rho", 20, 18, 16)
-    F_host = initial_data.build_field(rp, 13, cfg).ravel()
+    F_ref = oracle_field(rp, p_i, p_f, 13, cfg).ravel()
     F, _ = _device_field("rho", _engine.FP32, rp, p_i, p_f, cfg)
-    assert np.max(np.abs(F - F_host)) <= 2e-7  # FP32 storage rounding
+    assert np.max(np.abs(F - F_ref)) <= 2e-7  # FP32 storage rounding
     N = 20 * 18 * 16
     assert np.max(np.abs(F[4 * N:])) == 0.0  # rho components start at 0
 
@@ -61,11 +70,11 @@ def test_device_init_fp32_and_extra_components():
 def test_device_omega_from_chern_simons():
     cfg = dict(mode="rational_map", baryon_number=2, profile=("exp", 1.0), vacuum_band=True)
     mod, rp, p_i, p_f = make_params("omega", 22, 20, 18)
-    F_host = initial_data.build_field(rp, 5, cfg)
     c = 0.8
-    F_host[4] = -c * initial_data.chern_simons_density(F_host, rp)
+    F_ref = OIC.create_omega_field(oracle_field(rp, p_i, p_f, 5, cfg), p_i, p_f, c)
     F, _ = _device_field("omega", _engine.FP64, rp, p_i, p_f, cfg, omega_cs=c)
-    assert np.max(np.abs(F - F_host.ravel())) <= 1e-11 * max(1.0, np.max(np.abs(F_host)))
+    assert np.max(np.abs(F[4 * F_ref[0].size:])) > 0.1 * c  # omega is not trivially zero
+    assert np.max(np.abs(F - F_ref.ravel())) <= 1e-11 * max(1.0, np.max(np.abs(F_ref)))
 
 
 def test_batch_init_equals_engine_init():

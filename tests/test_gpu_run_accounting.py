@@ -80,3 +80,30 @@ def test_converged_run_stops_at_the_first_converged_step():
         st = e.anf_run(50, 1.0e9, 16)
     assert st.steps == 1 and st.converged == 1
     assert st.issued == 16 and st.graph_launches == 1
+
+
+def test_nonfinite_run_stops_at_the_same_step_on_every_path():
+    """An omega run with dt far beyond RK4's stability limit goes non-finite within a few steps (the
+    oracle at step 6).  A run without a tolerance enqueues its final energy pass right after its last
+    chunk; when a non-finite value stopped it earlier, it must still report SKX_ENONFINITE and leave the
+    handle on the field it stopped at, as a run with a tolerance (read after every chunk) and a run of
+    direct launches do."""
+    print("[case] non-finite omega run, three paths", flush=True)
+    mod, rp, p_i, p_f = make_params("omega", 16, 16, 16, time_step=0.6)
+    F = smooth_field(rp, 5, seed=1, normalise=True)
+    outs = []
+    for flags, tol in ((0, -1.0), (0, 0.0), (_engine.FLAG_NO_GRAPH, -1.0)):
+        with _engine.Engine(_engine.THEORY_OMEGA, _engine.FP64, p_i, p_f, flags=flags) as e:
+            e.upload(_engine.FIELD, F)
+            e.history_attach(32)
+            with pytest.raises(_engine.SkxError) as err:
+                e.anf_run(16, tol, 16)
+            assert err.value.code == -4
+            hist = e.history_read(32)
+            outs.append((len(hist), e.download(_engine.FIELD), e.download(_engine.VELOCITY)))
+    print("  steps before the stop:", [o[0] for o in outs], flush=True)
+    assert 0 < outs[0][0] < 16
+    for o in outs[1:]:
+        assert o[0] == outs[0][0]
+        assert np.array_equal(o[1], outs[0][1], equal_nan=True)
+        assert np.array_equal(o[2], outs[0][2], equal_nan=True)
