@@ -337,7 +337,10 @@ def main():
     if args.only in ("fp32", "fp64"):
         prec = _engine.FP32 if args.only == "fp32" else _engine.FP64
         r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk, trace=not args.no_trace)
-        print(f"[bench] {args.only}: {N * r['steps'] / r['seconds']:.4g} site-updates/s", file=sys.stderr, flush=True)
+        print(f"[bench] {args.only}: {N * r['steps'] / r['seconds']:.4g} site-updates/s, "
+              f"{1000.0 * r['seconds'] / r['steps']:.4f} ms/step", file=sys.stderr, flush=True)
+        for k, v in r.get("kernels", {}).items():
+            print(f"[bench]   {k}: {v['avg_us']:.2f} us x {v['launches']}", file=sys.stderr, flush=True)
         return
     copy_gbs, _ = _engine.stream_copy_bandwidth(1 << 30, 20)
     res = {}

@@ -142,6 +142,29 @@ __device__ __forceinline__ T stage_value(T f, T v, T l, T dt, T keep) {
   else return f;
 }
 
+// The stage field of one stage with its coefficients formed once per launch.
+// FP64 form: stage_value, the reference's operand order.  FP32 form:
+//   stage 2  f + (dt/2 keep) v,  stage 3  f + (dt/2 keep) v + (dt/4) l,  stage 4  f + (dt keep) v + (dt/2) l
+// as two fused multiply-adds (the same sums, rounded once per term).
+template <typename T, int STAGE>
+struct StageCoef {
+  T dt, keep, cv, cl;
+  __device__ StageCoef(T dt_, T keep_) : dt(dt_), keep(keep_) {
+    const T fv = STAGE == 4 ? T(1) : T(0.5), fl = STAGE == 4 ? T(0.5) : T(0.25);
+    cv = keep_ != T(0) ? fv * dt_ : T(0);
+    cl = fl * dt_;
+  }
+  __device__ __forceinline__ T operator()(T f, T v, T l) const {
+    if constexpr (sizeof(T) == 8 || STAGE <= 1) {
+      return stage_value<STAGE>(f, v, l, dt, keep);
+    } else if constexpr (STAGE == 2) {
+      return fma(cv, v, f);
+    } else {
+      return fma(cl, l, fma(cv, v, f));
+    }
+  }
+};
+
 // Buffer addressing: one resource per (array, component, plane) with the
 // plane's byte size as the range, so every load / store takes a 32-bit lane
 // offset and the scalar base; offsets at or beyond the range (OOB) read 0 and

@@ -148,11 +148,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   const T dt = T(A.P.dt);
   T keep = T(1);
   if constexpr (STAGE >= 2) keep = T(A.st->keep);
+  const StageCoef<T, STAGE> sc(dt, keep);
   AxisCoef<T> cx, cy, cz;
   cx.set(A.ilsx);
   cy.set(A.ilsy);
   cz.set(A.ilsz);
-  // per-thread guarded coefficients (the guards of core/derivatives.py:54,106,121 along y and z)
+  // per-thread guarded coefficients (the guards of core/derivatives.py:54,106,121 along y and z).
+  // The first differences along y and z are formed with their guards folded in, so they are 0 off
+  // the band at once; the mixed derivatives built from them carry stricter guards (i > halo), so an
+  // unguarded difference is never needed.
+  const T y1a = gy1 ? cy.d1a : T(0), y1b = gy1 ? cy.d1b : T(0);
+  const T z1a = gz1 ? cz.d1a : T(0), z1b = gz1 ? cz.d1b : T(0);
   const T y2a = gy1 ? cy.d2a : T(0), y2b = gy1 ? cy.d2b : T(0), y2c = gy1 ? cy.d2c : T(0);
   const T z2a = gz1 ? cz.d2a : T(0), z2b = gz1 ? cz.d2b : T(0), z2c = gz1 ? cz.d2c : T(0);
   const T mxy_a = gy2 ? cx.d1a : T(0), mxy_b = gy2 ? cx.d1b : T(0);  // x-part of d_x d_y, y guard folded
@@ -253,13 +259,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     constexpr int sm2 = (K + 1) % 5, sm1 = (K + 2) % 5, s0 = (K + 3) % 5, sp1 = (K + 4) % 5, sp2 = K;
     const int fs = (p + 8) & (FSLOTS - 1);  // p >= -2
 #pragma unroll
-    for (int a = 0; a < NST; ++a)
+    for (int j = 0; j < LPT; ++j) {
+      if (j == 0 || eidx[j] >= 0) {
 #pragma unroll
-      for (int j = 0; j < LPT; ++j) {
-        if (j == 0 || eidx[j] >= 0)
-          fring[fs][a][eidx[j]] = stage_value<STAGE>(pre[0][a][j], NA > 1 ? pre[NA > 1 ? 1 : 0][a][j] : T(0),
-                                                     NA > 2 ? pre[NA > 2 ? 2 : 0][a][j] : T(0), dt, keep);
+        for (int a = 0; a < NST; ++a)
+          fring[fs][a][eidx[j]] = sc(pre[0][a][j], NA > 1 ? pre[NA > 1 ? 1 : 0][a][j] : T(0),
+                                     NA > 2 ? pre[NA > 2 ? 2 : 0][a][j] : T(0));
       }
+    }
     T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
     if constexpr (RAWQ) {
       const int rs = p & 1;
@@ -318,15 +325,15 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     for (int a = 0; a < NST; ++a) {
       const T* t = fring[fs][a];
       qf[a][sp2] = t[C];
-      qy[a][sp2] = cd1(t[C - 2 * TW], t[C - TW], t[C + TW], t[C + 2 * TW], cy.d1a, cy.d1b);
-      const T dz = cd1(t[C - 2], t[C - 1], t[C + 1], t[C + 2], cz.d1a, cz.d1b);
+      qy[a][sp2] = cd1(t[C - 2 * TW], t[C - TW], t[C + TW], t[C + 2 * TW], y1a, y1b);
+      const T dz = cd1(t[C - 2], t[C - 1], t[C + 1], t[C + 2], z1a, z1b);
       qz[a][sp2] = dz;
       if constexpr (ANY_MIXED) {
         if ((Tr::MIXED >> a) & 1u) {
           zring[zs][a][(ty + 2) * TZ + tz] = dz;
           if (ty < 4) {
             const int h = hrow * TW + tz + 2;
-            zring[zs][a][hrow * TZ + tz] = cd1(t[h - 2], t[h - 1], t[h + 1], t[h + 2], cz.d1a, cz.d1b);
+            zring[zs][a][hrow * TZ + tz] = cd1(t[h - 2], t[h - 1], t[h + 1], t[h + 2], z1a, z1b);
           }
         }
       }
@@ -353,8 +360,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         const T f0 = qf[a][s0];
         phi[a] = f0;
         F[0][a] = cd1(qf[a][sm2], qf[a][sm1], qf[a][sp1], qf[a][sp2], x1a, x1b);
-        F[1][a] = gy1 ? qy[a][s0] : T(0);
-        F[2][a] = gz1 ? qz[a][s0] : T(0);
+        F[1][a] = qy[a][s0];
+        F[2][a] = qz[a][s0];
         if constexpr (D2) {
           const T* t = fring[cfs][a];
           S[0][a] = cd2(qf[a][sm2], qf[a][sm1], f0, qf[a][sp1], qf[a][sp2], x2a, x2b, x2c);
@@ -376,10 +383,10 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
       if constexpr (NF > NST) {
 #pragma unroll
         for (int a = NST; a < NF; ++a) {
-          if constexpr (STAGE == 4) phi[a] = stage_value<4>(cF[a], cV[a], cl2[a], dt, keep);
+          if constexpr (STAGE == 4) phi[a] = sc(cF[a], cV[a], cl2[a]);
           else
-            phi[a] = stage_value<STAGE>(cP[0][a - NST], NA > 1 ? cP[NA > 1 ? 1 : 0][a - NST] : T(0),
-                                        NA > 2 ? cP[NA > 2 ? 2 : 0][a - NST] : T(0), dt, keep);
+            phi[a] = sc(cP[0][a - NST], NA > 1 ? cP[NA > 1 ? 1 : 0][a - NST] : T(0),
+                        NA > 2 ? cP[NA > 2 ? 2 : 0][a - NST] : T(0));
         }
       }
       const size_t so = (size_t)c * YZ + own_off;
