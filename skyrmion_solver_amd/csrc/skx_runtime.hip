@@ -127,8 +127,8 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
 // per theory and stage from measurement at 128^3 (wide_tile_stage).  Where a
 // kernel holds 2 waves per SIMD on either tile, the taller tile cuts the
 // y-halo share of every plane from 1.69 to 1.41 loads per site (pion: stage 1
-// 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us); pion stages 2 and 3 keep 32x8,
-// where stage 2 runs 3 waves per SIMD and stage 3 measured faster.  A forced
+// 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us); pion stages 2 and 3 keep 32x8
+// (32x16 measured no faster for them).  A forced
 // tile_z and small lattices keep the base tile for every stage; a batch makes
 // the same choice for all its lattices at creation (skx_batch_create).
 //

@@ -115,11 +115,22 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   constexpr bool ANY_MIXED = D2 && (Tr::MIXED != 0u);
   constexpr int FSLOTS = 4;  // stage-field planes: lead p, p-1, centre p-2, and the slot being refilled
   constexpr int ZSLOTS = 3;  // raw z-differences for the yz mixed derivative: planes p, p-1, p-2
+  // lead-in planes whose loads are issued together (registers permitting: the
+  // FP64 stage 4 keeps two)
+  constexpr int LEAD_LOADS = (sizeof(T) == 8 && NA == 3) ? 2 : 4;
   static_assert(NT % 64 == 0, "block must be whole wavefronts");
   static_assert(TY >= 4, "halo rows of the z-difference ring are filled by the first four rows of threads");
 
-  __shared__ T fring[FSLOTS][NST][TE];  // (rawq: declared below, stage 4 only)
-  __shared__ T zring[ANY_MIXED ? ZSLOTS : 1][ANY_MIXED ? NST : 1][ANY_MIXED ? TR * TZ : 1];
+  // LDS layout of the rings.  With four stencil components (AOS) the
+  // components of one cell are contiguous, so a cell moves in one 16-byte
+  // (FP32) access and every neighbour is the thread's own cell address plus
+  // a constant; otherwise each component is a plane of its own.
+  constexpr bool AOS = (NST == 4);
+  constexpr int ZC = TR * TZ;  // cells of one z-difference plane (tile rows with halo x TZ)
+  __shared__ __align__(16) T fring[FSLOTS * NST * TE];  // (rawq: declared below, stage 4 only)
+  __shared__ __align__(16) T zring[ANY_MIXED ? ZSLOTS * NST * ZC : 1];
+  auto fi = [&](int slot, int a, int e) { return AOS ? (slot * TE + e) * NST + a : (slot * NST + a) * TE + e; };
+  auto zi = [&](int slot, int a, int e) { return AOS ? (slot * ZC + e) * NST + a : (slot * NST + a) * ZC + e; };
   __shared__ double red[2 * (NT / 64)];
 
   if ((A.flags & CHECK_DONE) && A.st->done) return;  // uniform per launch
@@ -205,13 +216,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   const int hrow = (ty < 2) ? ty : (TY + ty);
 
   const T* src[3] = {A.F, A.V, A.la};
-  T pre[NA][NST][LPT];
+  using PlaneBuf = T[NA][NST][LPT];
+  PlaneBuf pre;
   // Planes outside the lattice load as zeros: the descriptor points at a plane
   // inside it and every offset is out of range.  Every global load and store
   // of the march is issued on every path, so the compiler's memory-counter
   // waits are exact counts: a wait for one plane's loads never also waits for
   // later loads or for stores still in flight.
-  auto issue = [&](int p) {
+  auto issue = [&](PlaneBuf& buf, int p) {
     const bool pin = (p >= 0) && (p < X);
     const size_t pc = (size_t)min(max(p, 0), X - 1);
     uint32_t off[LPT];
@@ -223,7 +235,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
       for (int a = 0; a < NST; ++a) {
         const auto r = plane_rsrc(src[k] + a * N + pc * YZ, plane_bytes);
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) pre[k][a][j] = bload<T>(r, off[j]);
+        for (int j = 0; j < LPT; ++j) buf[k][a][j] = bload<T>(r, off[j]);
       }
   };
 
@@ -244,7 +256,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
                                2 * (NT / 64) * sizeof(double);
   constexpr size_t LDS_RAWQ = sizeof(T) * 2 * 3 * size_t(NST) * NT;
   constexpr bool RAWQ = (STAGE == 4) && (sizeof(T) == 4 || LDS_RINGS + LDS_RAWQ <= 160 * 1024);
-  __shared__ T rawq[RAWQ ? 2 : 1][RAWQ ? 3 * NST : 1][RAWQ ? NT : 1];
+  __shared__ __align__(16) T rawq[RAWQ ? 2 * NT * 3 * NST : 1];  // [slot][thread][F, V, l2 x NST]
 
   double acc0 = (STAGE == 4) ? -1.0e300 : 0.0;
   double acc1 = 0.0;
@@ -253,9 +265,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
 
   // One plane of the march.  K = phase mod 5: the lead plane p goes to queue
   // slot K; the centre c = p - 2 and its x-neighbours c-2..c+2 sit in slots
-  // (K+1)..(K+5) mod 5.
-  auto body = [&](auto Kc, int p) {
+  // (K+1)..(K+5) mod 5.  `buf` holds plane p's loads.  HAS_CENTRE = false
+  // compiles the centre site out (the four lead-in planes p < xs + 2, whose
+  // centre lies before the chunk); NEXT = true issues plane p + 1 into `pre`.
+  auto body = [&](auto Kc, auto HAS_CENTRE, auto NEXT, int p, PlaneBuf& buf) {
     constexpr int K = decltype(Kc)::value;
+    constexpr bool HC = decltype(HAS_CENTRE)::value;
     constexpr int sm2 = (K + 1) % 5, sm1 = (K + 2) % 5, s0 = (K + 3) % 5, sp1 = (K + 4) % 5, sp2 = K;
     const int fs = (p + 8) & (FSLOTS - 1);  // p >= -2
 #pragma unroll
@@ -263,33 +278,38 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
       if (j == 0 || eidx[j] >= 0) {
 #pragma unroll
         for (int a = 0; a < NST; ++a)
-          fring[fs][a][eidx[j]] = sc(pre[0][a][j], NA > 1 ? pre[NA > 1 ? 1 : 0][a][j] : T(0),
-                                     NA > 2 ? pre[NA > 2 ? 2 : 0][a][j] : T(0));
+          fring[fi(fs, a, eidx[j])] = sc(buf[0][a][j], NA > 1 ? buf[NA > 1 ? 1 : 0][a][j] : T(0),
+                                     NA > 2 ? buf[NA > 2 ? 2 : 0][a][j] : T(0));
       }
     }
     T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
     if constexpr (RAWQ) {
-      const int rs = p & 1;
+      T* rq = rawq + ((p & 1) * NT + tid) * (3 * NST);
+      if constexpr (HC) {
+#pragma unroll
+        for (int a = 0; a < NST; ++a) {
+          cF[a] = rq[a];
+          cV[a] = rq[NST + a];
+          cl2[a] = rq[2 * NST + a];
+        }
+      }
 #pragma unroll
       for (int a = 0; a < NST; ++a) {
-        cF[a] = rawq[rs][a][tid];
-        cV[a] = rawq[rs][NST + a][tid];
-        cl2[a] = rawq[rs][2 * NST + a][tid];
-        rawq[rs][a][tid] = pre[0][a][0];
-        rawq[rs][NST + a][tid] = pre[1][a][0];
-        rawq[rs][2 * NST + a][tid] = pre[NA > 2 ? 2 : 0][a][0];
+        rq[a] = buf[0][a][0];
+        rq[NST + a] = buf[1][a][0];
+        rq[2 * NST + a] = buf[NA > 2 ? 2 : 0][a][0];
       }
     }
 
     const int c = p - 2;
-    const bool center = (c >= xs) && own;
+    const bool center = HC && (c >= xs) && own;
     // centre-site loads, issued before the next plane's loads so that waiting
     // for them does not wait for those (offsets out of range before the chunk)
     const uint32_t coff = (c >= xs) ? own_b : kOOB;
     const size_t pb = (size_t)max(c, 0) * YZ;
     // stage 4: its own site's F, V, l1, l2, l3 for the RK4 combine -- F, V, l2
     // of the stencil components came from the raw ring above, the rest are loaded here
-    if constexpr (STAGE == 4) {
+    if constexpr (STAGE == 4 && HC) {
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
         const size_t o = a * N + pb;
@@ -305,35 +325,41 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     // site-only components (stages 0-3; stage 4 has them in cF / cV / cl2) and the curl-pass values
     T cP[NA][NF > NST ? NF - NST : 1];
     T cx_aux[NAUX > 0 ? NAUX : 1];
-    if constexpr (NF > NST && STAGE <= 3) {
+    if constexpr (NF > NST && STAGE <= 3 && HC) {
 #pragma unroll
       for (int k = 0; k < NA; ++k)
 #pragma unroll
         for (int a = NST; a < NF; ++a) cP[k][a - NST] = bload<T>(plane_rsrc(src[k] + a * N + pb, plane_bytes), coff);
     }
-    if constexpr (NAUX > 0 && STAGE >= 1) {
+    if constexpr (NAUX > 0 && STAGE >= 1 && HC) {
       // D^a_j at aux[3a + j] (the energy pass reads R^a_ij at aux[9 + 3a + p] below)
 #pragma unroll
       for (int q = 0; q < NAUX; ++q) cx_aux[q] = bload<T>(plane_rsrc(A.aux + q * N + pb, plane_bytes), coff);
     }
     __syncthreads();
-    issue(p + 1);  // past the end of the chunk: a plane inside the lattice or zeros, never read
+    if constexpr (decltype(NEXT)::value) issue(pre, p + 1);  // past the end of the chunk: never read
 
     // lead plane p: own-column value and raw first differences along y and z
     const int zs = (p + 9) % ZSLOTS;
 #pragma unroll
     for (int a = 0; a < NST; ++a) {
-      const T* t = fring[fs][a];
-      qf[a][sp2] = t[C];
-      qy[a][sp2] = cd1(t[C - 2 * TW], t[C - TW], t[C + TW], t[C + 2 * TW], y1a, y1b);
-      const T dz = cd1(t[C - 2], t[C - 1], t[C + 1], t[C + 2], z1a, z1b);
+      auto t = [&](int e) { return fring[fi(fs, a, e)]; };
+      qf[a][sp2] = t(C);
+      qy[a][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), y1a, y1b);
+      const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), z1a, z1b);
       qz[a][sp2] = dz;
       if constexpr (ANY_MIXED) {
-        if ((Tr::MIXED >> a) & 1u) {
-          zring[zs][a][(ty + 2) * TZ + tz] = dz;
-          if (ty < 4) {
-            const int h = hrow * TW + tz + 2;
-            zring[zs][a][hrow * TZ + tz] = cd1(t[h - 2], t[h - 1], t[h + 1], t[h + 2], z1a, z1b);
+        if ((Tr::MIXED >> a) & 1u) zring[zi(zs, a, (ty + 2) * TZ + tz)] = dz;
+      }
+    }
+    if constexpr (ANY_MIXED) {
+      if (ty < 4) {  // the halo rows of the z-difference plane
+        const int h = hrow * TW + tz + 2;
+#pragma unroll
+        for (int a = 0; a < NST; ++a) {
+          if ((Tr::MIXED >> a) & 1u) {
+            auto t = [&](int e) { return fring[fi(fs, a, e)]; };
+            zring[zi(zs, a, hrow * TZ + tz)] = cd1(t(h - 2), t(h - 1), t(h + 1), t(h + 2), z1a, z1b);
           }
         }
       }
@@ -363,17 +389,16 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         F[1][a] = qy[a][s0];
         F[2][a] = qz[a][s0];
         if constexpr (D2) {
-          const T* t = fring[cfs][a];
+          auto t = [&](int e) { return fring[fi(cfs, a, e)]; };
           S[0][a] = cd2(qf[a][sm2], qf[a][sm1], f0, qf[a][sp1], qf[a][sp2], x2a, x2b, x2c);
-          S[1][a] = cd2(t[C - 2 * TW], t[C - TW], f0, t[C + TW], t[C + 2 * TW], y2a, y2b, y2c);
-          S[2][a] = cd2(t[C - 2], t[C - 1], f0, t[C + 1], t[C + 2], z2a, z2b, z2c);
+          S[1][a] = cd2(t(C - 2 * TW), t(C - TW), f0, t(C + TW), t(C + 2 * TW), y2a, y2b, y2c);
+          S[2][a] = cd2(t(C - 2), t(C - 1), f0, t(C + 1), t(C + 2), z2a, z2b, z2c);
           if ((Tr::MIXED >> a) & 1u) {
             S[3][a] = cd1(qy[a][sm2], qy[a][sm1], qy[a][sp1], qy[a][sp2], xya, xyb);
             S[4][a] = cd1(qz[a][sm2], qz[a][sm1], qz[a][sp1], qz[a][sp2], xza, xzb);
             if constexpr (ANY_MIXED) {
-              const T* zr = zring[czs][a];
-              S[5][a] = cd1(zr[ty * TZ + tz], zr[(ty + 1) * TZ + tz], zr[(ty + 3) * TZ + tz], zr[(ty + 4) * TZ + tz],
-                            myz_a, myz_b);
+              auto zr = [&](int row) { return zring[zi(czs, a, row * TZ + tz)]; };
+              S[5][a] = cd1(zr(ty), zr(ty + 1), zr(ty + 3), zr(ty + 4), myz_a, myz_b);
             }
           } else {
             S[3][a] = S[4][a] = S[5][a] = T(0);
@@ -496,11 +521,11 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         }
       }
     }
-    if constexpr (D2 && STAGE <= 3) {
+    if constexpr (HC && D2 && STAGE <= 3) {
 #pragma unroll
       for (int a = 0; a < NF; ++a) bstore<T>(lv[a], plane_rsrc(A.lout + a * N + pb, plane_bytes), coff);
     }
-    if constexpr (STAGE == 4) {
+    if constexpr (HC && STAGE == 4) {
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
         bstore<T>(Fn[a], plane_rsrc(A.Fout + a * N + pb, plane_bytes), coff);
@@ -509,18 +534,39 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     }
   };
 
-  issue(xs - 2);
+  // Lead-in: the planes xs-2 .. xs+1 fill the rings before the first centre.
+  // Their loads (the first LEAD_LOADS of them at once) are in flight
+  // together, so the march starts after about one memory latency rather
+  // than four; from plane xs+2 on, each plane's loads are issued one plane
+  // ahead.  A chunk holds at least one plane, so xs+2 <= xe+1.
+  PlaneBuf pre1, pre2, pre3;
+  issue(pre, xs - 2);
+  if constexpr (LEAD_LOADS > 1) issue(pre1, xs - 1);
+  if constexpr (LEAD_LOADS > 2) issue(pre2, xs);
+  if constexpr (LEAD_LOADS > 3) issue(pre3, xs + 1);
+  using NoCentre = std::false_type;
+  auto lead_in = [&](auto Ic, PlaneBuf& own_buf) {
+    constexpr int i = decltype(Ic)::value;
+    if constexpr (i < LEAD_LOADS) body(Ic, NoCentre{}, std::bool_constant<i == LEAD_LOADS - 1>{}, xs - 2 + i, own_buf);
+    else body(Ic, NoCentre{}, std::true_type{}, xs - 2 + i, pre);
+  };
+  lead_in(IC<0>{}, pre);
+  lead_in(IC<1>{}, pre1);
+  lead_in(IC<2>{}, pre2);
+  lead_in(IC<3>{}, pre3);
   const int pend = xe + 1;
-  for (int p = xs - 2;;) {
-    body(IC<0>{}, p);
+  using Centre = std::true_type;
+  using Next = std::true_type;
+  for (int p = xs + 2;;) {
+    body(IC<4>{}, Centre{}, Next{}, p, pre);
     if (++p > pend) break;
-    body(IC<1>{}, p);
+    body(IC<0>{}, Centre{}, Next{}, p, pre);
     if (++p > pend) break;
-    body(IC<2>{}, p);
+    body(IC<1>{}, Centre{}, Next{}, p, pre);
     if (++p > pend) break;
-    body(IC<3>{}, p);
+    body(IC<2>{}, Centre{}, Next{}, p, pre);
     if (++p > pend) break;
-    body(IC<4>{}, p);
+    body(IC<3>{}, Centre{}, Next{}, p, pre);
     if (++p > pend) break;
   }
 
@@ -572,16 +618,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   }
 }
 
-// Waves per SIMD the register allocation must leave room for: 3 for the
-// FP32 pion stage 2, which fits in 168 VGPRs without spilling (three 46 KB
-// blocks per CU); no constraint elsewhere (the others would spill).
-template <typename T, int TH, int STAGE>
-constexpr int stage_min_waves() {
-  return (sizeof(T) == 4 && STAGE == 2 && TH == PION) ? 3 : 1;
-}
-
 template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY) __attribute__((amdgpu_waves_per_eu(stage_min_waves<T, TH, STAGE>())))
+__global__ void __launch_bounds__(TZ* TY)
 k_stage(const StageArgs<T> A) {
   stage_body<T, TH, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
 }
@@ -591,7 +629,7 @@ k_stage(const StageArgs<T> A) {
 // to a slot (-1 = idle block); the slot carries the lattice's arguments for
 // both ping-pong parities and its run of blocks in each map.
 template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY) __attribute__((amdgpu_waves_per_eu(stage_min_waves<T, TH, STAGE>())))
+__global__ void __launch_bounds__(TZ* TY)
 k_stage_batch(const BatchSlot<T>* __restrict__ slots, const int* __restrict__ blk_inst, int map, int parity) {
   const int inst = blk_inst[blockIdx.x];
   if (inst < 0) return;
