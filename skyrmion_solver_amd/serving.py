@@ -292,11 +292,17 @@ def main(argv=None):
     ap.add_argument("--refine", action="store_true",
                     help="FP32: finish each lattice in FP64 once its max|g| is below --refine-factor x tol")
     ap.add_argument("--refine-factor", type=float, default=3.0)
+    ap.add_argument("--omega-factor", type=float, default=workloads.STEP_BUDGET_FACTOR["omega"],
+                    help="step limit of omega requests as a multiple of --max-steps")
+    ap.add_argument("--omega-max-size", type=int, default=0, help="largest omega lattice (0: no cap)")
     args = ap.parse_args(argv)
+    factors = dict(workloads.STEP_BUDGET_FACTOR, omega=args.omega_factor)
+    caps = {"omega": args.omega_max_size} if args.omega_max_size > 0 else None
     prec = _engine.FP32 if args.precision == "fp32" else _engine.FP64
     for rate in args.rates:
         reqs = workloads.c5_requests(n=args.n, rate=rate, seed=args.seed, sizes=args.sizes, tol=args.tol,
-                                     max_steps=args.max_steps, device_init=True)
+                                     max_steps=args.max_steps, device_init=True, max_size_by_theory=caps,
+                                     budget_factor=factors)
         srv = Server(precision=prec, hbm_budget_bytes=int(args.hbm_gib * (1 << 30)), chunk_steps=args.chunk,
                      refine_fp64=args.refine, refine_factor=args.refine_factor)
         rep = srv.serve(reqs, progress_every=args.progress)

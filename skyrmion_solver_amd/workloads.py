@@ -26,8 +26,10 @@ THEORY_NAMES = {"pion": "Nuclear Skyrme model", "omega": "Omega-Skyrme model", "
                 "rho": "Rho-Skyrme model", "berger": "Berger Skyrme model", "lightly": "Lightly bound Skyrme model"}
 SERVE_MIX = (("pion", 0.60), ("omega", 0.25), ("rho", 0.15))
 # step budget of a serving request per theory, as a multiple of the stream's max_steps: the omega saddle
-# flow (pion field descending while omega ascends) relaxes in 10k-38k steps at 64^3, B = 1..4
-STEP_BUDGET_FACTOR = {"pion": 1, "omega": 4, "rho": 1}
+# flow (pion field descending while omega ascends) relaxes slowly.  Measured at 64^3 (seed 0, FP32 /
+# FP64 steps to max|g| < 1e-4): B = 1 9.6k / 10.2k, B = 2 22.6k / 23.5k, B = 3 17.3k / 20.2k,
+# B = 4 48.4k / 66.4k, B = 5 23.3k / 21.9k, B = 6 73.1k / 65.3k, B = 7 79.3k / > 80k, B = 8 > 80k.
+STEP_BUDGET_FACTOR = {"pion": 1, "omega": 32, "rho": 1}
 
 
 def _lattice(L, dx, courant=COURANT):
@@ -178,9 +180,14 @@ def serving_request(rid: int, theory: str, L: int, B: int, rng, tol: float = 1.0
 
 
 def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 128, 160, 192), mix=SERVE_MIX,
-                tol: float = 1.0e-4, max_steps: int = 20000, device_init: bool = False):
+                tol: float = 1.0e-4, max_steps: int = 20000, device_init: bool = False, max_size_by_theory=None,
+                budget_factor=None):
     """Poisson stream of `n` requests at `rate` per second: theory drawn from `mix`, L from `sizes`, B = 1..8;
-    each request's step limit is `max_steps` times its theory's ``STEP_BUDGET_FACTOR``."""
+    each request's step limit is `max_steps` times its theory's factor in `budget_factor` (default
+    ``STEP_BUDGET_FACTOR``).  `max_size_by_theory` ({theory: L_max}) caps a theory's lattice size without
+    changing the random draws, so every other request of the stream stays the same."""
+    factors = STEP_BUDGET_FACTOR if budget_factor is None else budget_factor
+    caps = max_size_by_theory or {}
     from skyrmion_solver_amd.serving import poisson_arrivals
 
     rng = np.random.default_rng(seed)
@@ -192,8 +199,10 @@ def c5_requests(n: int = 600, rate: float = 20.0, seed: int = 0, sizes=(64, 96, 
     for k in range(n):
         theory = str(rng.choice(names, p=probs))
         L = int(rng.choice(sizes))
+        if theory in caps:
+            L = min(L, int(caps[theory]))
         B = int(rng.integers(1, 9))
         out.append(serving_request(k, theory, L, B, rng, tol=tol,
-                                   max_steps=int(max_steps * STEP_BUDGET_FACTOR.get(theory, 1)),
+                                   max_steps=int(max_steps * factors.get(theory, 1)),
                                    arrival=float(arrivals[k]), device_init=device_init))
     return out
