@@ -63,8 +63,10 @@ enum skx_density_kind {
 
 enum skx_flags {
   SKX_FLAG_NO_GRAPH = 1, /* launch stage kernels directly instead of replaying a captured hipGraph */
-  SKX_FLAG_TRACE = 2     /* time every stage kernel with a pair of HIP events on the handle's stream
+  SKX_FLAG_TRACE = 2,    /* time every stage kernel with a pair of HIP events on the handle's stream
                             (runs then launch directly, without graphs) */
+  SKX_FLAG_ONE_LANE = 4  /* FP64 pion: RK stages 2-4 with one thread per site instead of two (the
+                            default wherever N + Y*Z < 2^28 sites); results agree to rounding */
 };
 
 /* Lattice + theory description.  p_i / p_f are the reference's packed

@@ -31,6 +31,8 @@ FP64, FP32 = 0, 1
 FIELD, VELOCITY = 0, 1
 DENSITY_ENERGY, DENSITY_BARYON, DENSITY_POTENTIAL, DENSITY_RHO = 0, 1, 2, 3
 FLAG_NO_GRAPH, FLAG_TRACE = 1, 2
+# FP64 pion: RK4 stage kernels with one thread per site (the default is two wherever the lattice allows)
+FLAG_ONE_LANE = 4
 LUMP_HEDGEHOG, LUMP_RATIONAL_MAP = 0, 1
 PROFILE_EXP, PROFILE_GAUSS, PROFILE_FIT = 0, 1, 2
 MAX_LUMPS = 16

@@ -115,6 +115,61 @@ __device__ bool grid_reduce2(double& r0, double& r1, double* partial, uint32_t* 
   return true;
 }
 
+// The end of a stage launch, after every block's march: stages 0 / 1 reduce
+// the energy and baryon sums of the incoming field and take the pending
+// arrest decision; stage 4 reduces the signed max and max|g| (+inf marks a
+// non-finite gradient) and counts the step.  Thread 0 of the last block to
+// finish updates the lattice's DevStatus.
+template <int NT, int STAGE, typename T>
+__device__ __forceinline__ void stage_epilogue(const StageArgs<T>& A, double acc0, double acc1, bool bad,
+                                               double* red, int blk, int nblk) {
+  if constexpr (STAGE <= 1) {
+    if (STAGE == 1 && !(A.flags & (FUSE_ENERGY | FUSE_BARYON))) return;
+    double r0 = acc0, r1 = acc1;
+    if (grid_reduce2<NT, false>(r0, r1, A.partial, &A.st->cnt_sum, red, blk, nblk)) {
+      DevStatus* st = A.st;
+      if (A.flags & FUSE_ENERGY) st->energy = r0;
+      if (A.flags & FUSE_BARYON) st->baryon = r1;
+      if (!isfinite(r0) || !isfinite(r1)) {
+        st->nonfinite = 1;
+        st->done = 1;
+      }
+      if (A.flags & DECIDE_ARREST) {
+        double kp = 1.0;
+        if (st->arrest_pending && st->killkinen && (r0 > st->e_prev)) kp = 0.0;
+        st->keep = kp;
+        st->arrest_pending = 0;
+        st->e_prev = r0;
+      }
+      if (A.hist && (A.flags & DECIDE_ARREST) && st->steps < st->hist_cap) {
+        A.hist[st->steps].energy = r0;
+        A.hist[st->steps].baryon = r1;
+      }
+    }
+  } else if constexpr (STAGE == 4) {
+    // +inf in the max|g| slot marks a non-finite gradient
+    double r0 = acc0, r1 = bad ? __longlong_as_double(0x7ff0000000000000LL) : acc1;
+    if (grid_reduce2<NT, true>(r0, r1, A.partial, &A.st->cnt_max, red, blk, nblk)) {
+      DevStatus* st = A.st;
+      st->err_signed = r0;
+      st->err_abs = r1;
+      if (!isfinite(r1)) {
+        st->nonfinite = 1;
+        st->done = 1;
+      }
+      if (A.flags & RUN_MODE) {
+        if (A.hist && st->steps < st->hist_cap) {
+          A.hist[st->steps].err_signed = r0;
+          A.hist[st->steps].err_abs = r1;
+        }
+        st->steps += 1;
+        st->arrest_pending = 1;
+        if (r1 < st->tol || st->steps >= st->max_steps) st->done = 1;
+      }
+    }
+  }
+}
+
 template <int K>
 using IC = std::integral_constant<int, K>;
 

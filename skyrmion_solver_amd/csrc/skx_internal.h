@@ -93,9 +93,18 @@ struct BatchSlot {
   int pad_[3];
 };
 
+// The two-lanes-per-site stage kernels (skx_stage_pair.h) address a lane's
+// second component as an offset of N elements past the first one's buffer
+// descriptor; they apply where that offset, plus one plane, stays below the
+// 2^31-byte range of a buffer offset.
+inline bool lane_pair_fits(long long N, long long YZ, size_t elem) {
+  return (unsigned long long)(N + YZ) * elem < (1ull << 31);
+}
+
 // Kernel launchers (skx_kernels.hip).  Return hipSuccess or the launch error.
 struct LaunchShape {
   int tz, ty, xchunk;
+  int lanes = 1;  // threads per site: 2 for the FP64 pion stage kernels of skx_stage_pair.h
 };
 
 hipError_t launch_stage_f64(int theory, int stage, const StageArgs<double>& a, const LaunchShape& s, hipStream_t st);

@@ -163,6 +163,14 @@ int choose_xchunk(const Lattice& L, const LaunchShape& tile, int blocks_per_cu, 
 
 size_t elem_size(int prec) { return prec == SKX_FP64 ? sizeof(double) : sizeof(float); }
 
+// The FP64 pion RK stages 2-4 run two threads per site (skx_stage_pair.h): two
+// waves per SIMD where the one-lane kernels hold one, and no register spills.
+bool pair_theory(int theory, int prec) { return prec == SKX_FP64 && theory == SKX_THEORY_PION; }
+bool use_lane_pairs(const Lattice& L, uint32_t flags) {
+  return pair_theory(L.theory, L.prec) && !(flags & SKX_FLAG_ONE_LANE) &&
+         lane_pair_fits(L.N, (long long)L.Y * L.Z, elem_size(L.prec));
+}
+
 }  // namespace
 
 struct skx_engine {
@@ -669,7 +677,8 @@ int create_engine(const skx_desc* desc, hipStream_t shared, DevStatus* ext_st, D
     return fail(SKX_EHIP, std::string("hipDeviceGetAttribute: ") + hipGetErrorString(r));
   }
   for (int stg = 0; stg < 5; ++stg) {
-    const LaunchShape st = stage_tile(L, tile, stg, per_stage);
+    LaunchShape st = stage_tile(L, tile, stg, per_stage);
+    if (stg >= 2 && use_lane_pairs(L, desc->flags)) st.lanes = 2;
     int bpc = 0;
     if ((r = stage_blocks_per_cu(L.prec == SKX_FP32, L.theory, stg, st, &bpc)) != hipSuccess) {
       delete e;
@@ -1165,6 +1174,8 @@ int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   for (int stage = 1; stage <= 4; ++stage) {
     const bool wide = d->tile_z == 0 && d->precision == SKX_FP32 && wide_tile_stage(d->theory, stage);
     b->stile[stage] = wide ? kWideTile : b->tile;
+    if (stage >= 2 && pair_theory(d->theory, d->precision) && !(d->flags & SKX_FLAG_ONE_LANE))
+      b->stile[stage].lanes = 2;
     b->smap[stage] = wide ? 1 : 0;
     if (wide) b->nmaps = 2;
   }
@@ -1219,6 +1230,9 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   if (i == b->max_slots) return fail(SKX_EBUSY, "no free slot in the batch");
   Lattice L;
   if (int rc = parse_desc(lattice, L)) return rc;
+  if (b->stile[4].lanes == 2 && !lane_pair_fits(L.N, (long long)L.Y * L.Z, elem_size(L.prec)))
+    return fail(SKX_EINVAL, "the lattice is too large for the two-lanes-per-site FP64 kernels of this batch "
+                            "(N + Y*Z must stay below 2^28 sites); create the batch with SKX_FLAG_ONE_LANE");
   // x-chunk: the whole lattice unless the caller asks for shorter marches
   // (the batch's other lattices supply the parallelism)
   const int xc = (lattice->x_chunk > 0 && lattice->x_chunk < L.X) ? lattice->x_chunk : L.X;
@@ -1241,6 +1255,7 @@ int skx_batch_add(skx_batch_handle b, const skx_desc* lattice, const double* F0,
   }
   skx_desc d = *lattice;
   d.tile_z = b->tile.tz;
+  d.flags = (d.flags & ~(uint32_t)SKX_FLAG_ONE_LANE) | (b->flags & SKX_FLAG_ONE_LANE);
   skx_engine* e = nullptr;
   if (int rc = create_engine(&d, b->stream, b->d_status + i, b->h_status + i, xc, b->nmaps > 1, &e)) return rc;
   const size_t n = (size_t)L.N * L.nf;

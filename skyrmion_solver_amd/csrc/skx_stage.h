@@ -29,6 +29,7 @@
 
 #include "skx_device.h"
 #include "skx_rho_curl.h"
+#include "skx_stage_pair.h"
 
 namespace skx {
 
@@ -571,71 +572,40 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   }
 
   // ---- fused reductions
-  if constexpr (STAGE <= 1) {
-    if (STAGE == 1 && !(A.flags & (FUSE_ENERGY | FUSE_BARYON))) return;
-    double r0 = acc0, r1 = acc1;
-    if (grid_reduce2<NT, false>(r0, r1, A.partial, &A.st->cnt_sum, red, blk, nblk)) {
-      DevStatus* st = A.st;
-      if (A.flags & FUSE_ENERGY) st->energy = r0;
-      if (A.flags & FUSE_BARYON) st->baryon = r1;
-      if (!isfinite(r0) || !isfinite(r1)) {
-        st->nonfinite = 1;
-        st->done = 1;
-      }
-      if (A.flags & DECIDE_ARREST) {
-        double kp = 1.0;
-        if (st->arrest_pending && st->killkinen && (r0 > st->e_prev)) kp = 0.0;
-        st->keep = kp;
-        st->arrest_pending = 0;
-        st->e_prev = r0;
-      }
-      if (A.hist && (A.flags & DECIDE_ARREST) && st->steps < st->hist_cap) {
-        A.hist[st->steps].energy = r0;
-        A.hist[st->steps].baryon = r1;
-      }
-    }
-  } else if constexpr (STAGE == 4) {
-    // +inf in the max|g| slot marks a non-finite gradient
-    double r0 = acc0, r1 = bad ? __longlong_as_double(0x7ff0000000000000LL) : acc1;
-    if (grid_reduce2<NT, true>(r0, r1, A.partial, &A.st->cnt_max, red, blk, nblk)) {
-      DevStatus* st = A.st;
-      st->err_signed = r0;
-      st->err_abs = r1;
-      if (!isfinite(r1)) {
-        st->nonfinite = 1;
-        st->done = 1;
-      }
-      if (A.flags & RUN_MODE) {
-        if (A.hist && st->steps < st->hist_cap) {
-          A.hist[st->steps].err_signed = r0;
-          A.hist[st->steps].err_abs = r1;
-        }
-        st->steps += 1;
-        st->arrest_pending = 1;
-        if (r1 < st->tol || st->steps >= st->max_steps) st->done = 1;
-      }
-    }
-  }
+  stage_epilogue<NT, STAGE>(A, acc0, acc1, bad, red, blk, nblk);
 }
 
-template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY)
+// Stage kernels with two lanes per site (skx_stage_pair.h): RK stages 2-4 of
+// the FP64 pion theory.  LANES = 2 selects them; the launcher falls back to
+// one lane per site where the lattice's offsets do not fit (lane_pair_fits) or
+// the caller asks for it (SKX_FLAG_ONE_LANE).
+template <typename T, int TH, int STAGE>
+constexpr bool has_lane_pairs() {
+  return std::is_same<T, double>::value && TH == PION && STAGE >= 2;
+}
+
+template <typename T, int TH, int STAGE, int TZ, int TY, int LANES>
+__global__ void __launch_bounds__((TZ * TY * LANES))
 k_stage(const StageArgs<T> A) {
-  stage_body<T, TH, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
+  if constexpr (LANES == 2) stage_body_pair<T, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
+  else stage_body<T, TH, STAGE, TZ, TY>(A, blockIdx.x, gridDim.x);
 }
 
 // Batched launch: many lattices of one theory / precision / tile shape in one
 // grid.  blk_inst (block map `map` of the batch) maps every block of the grid
 // to a slot (-1 = idle block); the slot carries the lattice's arguments for
 // both ping-pong parities and its run of blocks in each map.
-template <typename T, int TH, int STAGE, int TZ, int TY>
-__global__ void __launch_bounds__(TZ* TY)
+template <typename T, int TH, int STAGE, int TZ, int TY, int LANES>
+__global__ void __launch_bounds__((TZ * TY * LANES))
 k_stage_batch(const BatchSlot<T>* __restrict__ slots, const int* __restrict__ blk_inst, int map, int parity) {
   const int inst = blk_inst[blockIdx.x];
   if (inst < 0) return;
   const BatchSlot<T>& sl = slots[inst];
   if (!sl.active) return;
-  if constexpr (STAGE >= 1)
+  if constexpr (STAGE >= 1 && LANES == 2)
+    stage_body_pair<T, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], (int)blockIdx.x - sl.first_block[map],
+                                      sl.nblocks[map]);
+  else if constexpr (STAGE >= 1)
     stage_body<T, TH, STAGE, TZ, TY>(sl.args[STAGE - 1][parity], (int)blockIdx.x - sl.first_block[map],
                                      sl.nblocks[map]);
 }
@@ -683,7 +653,13 @@ struct StageLaunchFn {
       const hipError_t r = hipGetLastError();
       if (r != hipSuccess) return r;
     }
-    hipLaunchKernelGGL((k_stage<T, TH, STAGE, TZ, TY>), dim3(blocks), dim3(TZ * TY), 0, st, b);
+    if constexpr (has_lane_pairs<T, TH, STAGE>()) {
+      if (s.lanes == 2) {
+        hipLaunchKernelGGL((k_stage<T, TH, STAGE, TZ, TY, 2>), dim3(blocks), dim3(2 * TZ * TY), 0, st, b);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_stage<T, TH, STAGE, TZ, TY, 1>), dim3(blocks), dim3(TZ * TY), 0, st, b);
     return hipGetLastError();
   }
 };
@@ -692,7 +668,7 @@ template <typename T, int TH>
 struct StageBatchLaunchFn {
   const BatchSlot<T>* slots;
   const int* blk_inst;
-  int map, parity, grid;
+  int map, parity, grid, lanes;
   hipStream_t st;
   template <int STAGE, int TZ, int TY>
   hipError_t operator()() {
@@ -702,7 +678,14 @@ struct StageBatchLaunchFn {
       const hipError_t r = hipGetLastError();
       if (r != hipSuccess) return r;
     }
-    hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY>), dim3(grid), dim3(TZ * TY), 0, st, slots, blk_inst, map,
+    if constexpr (has_lane_pairs<T, TH, STAGE>()) {
+      if (lanes == 2) {
+        hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY, 2>), dim3(grid), dim3(2 * TZ * TY), 0, st, slots,
+                           blk_inst, map, parity);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_stage_batch<T, TH, STAGE, TZ, TY, 1>), dim3(grid), dim3(TZ * TY), 0, st, slots, blk_inst, map,
                        parity);
     return hipGetLastError();
   }
@@ -711,10 +694,16 @@ struct StageBatchLaunchFn {
 template <typename T, int TH>
 struct StageOccupancyFn {
   int* out;
+  int lanes;
   template <int STAGE, int TZ, int TY>
   hipError_t operator()() {
+    if constexpr (has_lane_pairs<T, TH, STAGE>()) {
+      if (lanes == 2)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            out, reinterpret_cast<const void*>(&k_stage<T, TH, STAGE, TZ, TY, 2>), 2 * TZ * TY, 0);
+    }
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        out, reinterpret_cast<const void*>(&k_stage<T, TH, STAGE, TZ, TY>), TZ * TY, 0);
+        out, reinterpret_cast<const void*>(&k_stage<T, TH, STAGE, TZ, TY, 1>), TZ * TY, 0);
   }
 };
 
@@ -726,12 +715,12 @@ hipError_t stage_launch(int stage, const StageArgs<T>& a, const LaunchShape& s, 
 template <typename T, int TH>
 hipError_t stage_launch_batch(int stage, const LaunchShape& s, const BatchSlot<T>* slots, const int* blk_inst,
                               int map, int parity, int grid, hipStream_t st) {
-  return visit_shape<T, TH>(stage, s, StageBatchLaunchFn<T, TH>{slots, blk_inst, map, parity, grid, st});
+  return visit_shape<T, TH>(stage, s, StageBatchLaunchFn<T, TH>{slots, blk_inst, map, parity, grid, s.lanes, st});
 }
 
 template <typename T, int TH>
 hipError_t stage_occupancy(int stage, const LaunchShape& s, int* blocks) {
-  return visit_shape<T, TH>(stage, s, StageOccupancyFn<T, TH>{blocks});
+  return visit_shape<T, TH>(stage, s, StageOccupancyFn<T, TH>{blocks, s.lanes});
 }
 
 // One translation unit per (precision, theory) instantiates these.
