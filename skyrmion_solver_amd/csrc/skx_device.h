@@ -197,6 +197,95 @@ __device__ __forceinline__ T cd2(T fm2, T fm1, T f0, T fp1, T fp2, T a, T b, T c
   return a * (fm1 + fp1) - b * (fm2 + fp2) - c * f0;
 }
 
+// Position of one site of a stage kernel's tile: the block's (y, z) tile and
+// x-chunk, with the XCD-aware block order (consecutive tiles, which share y
+// halos, on the same XCD's L2), and the site `s` of the TZ x TY tile.
+struct TileSite {
+  int z0, y0;  // tile origin
+  int xs, xe;  // planes [xs, xe) of the block's x-chunk
+  int ty, tz;  // site within the tile
+  int y, z;    // lattice coordinates of the site
+  bool own;    // the site lies in the lattice
+};
+template <int TZ, int TY, typename T>
+__device__ __forceinline__ TileSite tile_site(const StageArgs<T>& A, int blk, int nblk, int s) {
+  TileSite t;
+  t.tz = s % TZ;
+  t.ty = s / TZ;
+  int lin = blk;
+  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
+  const int bz = lin % A.nbz;
+  lin /= A.nbz;
+  const int by = lin % A.nby;
+  const int bx = lin / A.nby;
+  t.z0 = bz * TZ;
+  t.y0 = by * TY;
+  t.xs = bx * A.xchunk;
+  t.xe = min(t.xs + A.xchunk, A.X);
+  t.y = t.y0 + t.ty;
+  t.z = t.z0 + t.tz;
+  t.own = (t.y < A.Y) && (t.z < A.Z);
+  return t;
+}
+
+// Halo cell h of a tile with TR rows of TW cells (2-cell halo): rows 0, 1,
+// TR-2, TR-1 in full, then columns 0, 1, TW-2, TW-1 of the rows between.
+template <int TW, int TR>
+__device__ __forceinline__ void halo_cell(int h, int& r, int& cc) {
+  if (h < 2 * TW) {
+    r = h / TW;
+    cc = h - r * TW;
+  } else if (h < 4 * TW) {
+    const int m = h - 2 * TW;
+    r = TR - 2 + m / TW;
+    cc = m % TW;
+  } else {
+    const int m = h - 4 * TW, q = m & 3;
+    r = 2 + m / 4;
+    cc = q < 2 ? q : TW - 4 + q;
+  }
+}
+
+// Stencil coefficients of one site with the guards of core/derivatives.py:54,
+// 106,121 folded in along y and z.  The first differences along y and z are
+// formed with their guards (halo <= i < n - halo) in the coefficients, so
+// they are 0 off the band at once; the mixed derivatives built from them carry
+// the stricter guards (halo < i < n - halo), so an unguarded difference is
+// never needed.  The x guards change per plane and are applied there.
+template <typename T>
+struct SiteCoef {
+  AxisCoef<T> cx, cy, cz;
+  T y1a, y1b, z1a, z1b;           // first differences along y, z
+  T y2a, y2b, y2c, z2a, z2b, z2c; // pure second differences along y, z
+  T mxy_a, mxy_b;                 // x-part of d_x d_y, y guard folded
+  T mxz_a, mxz_b;                 // x-part of d_x d_z, z guard folded
+  T myz_a, myz_b;                 // y-part of d_y d_z, y and z guards folded
+  __device__ __forceinline__ SiteCoef(const StageArgs<T>& A, int y, int z) {
+    const int Y = A.Y, Z = A.Z, halo = A.halo;
+    const bool gy1 = (y >= halo) && (y < Y - halo), gz1 = (z >= halo) && (z < Z - halo);
+    const bool gy2 = (y > halo) && (y < Y - halo), gz2 = (z > halo) && (z < Z - halo);
+    cx.set(A.ilsx);
+    cy.set(A.ilsy);
+    cz.set(A.ilsz);
+    y1a = gy1 ? cy.d1a : T(0);
+    y1b = gy1 ? cy.d1b : T(0);
+    z1a = gz1 ? cz.d1a : T(0);
+    z1b = gz1 ? cz.d1b : T(0);
+    y2a = gy1 ? cy.d2a : T(0);
+    y2b = gy1 ? cy.d2b : T(0);
+    y2c = gy1 ? cy.d2c : T(0);
+    z2a = gz1 ? cz.d2a : T(0);
+    z2b = gz1 ? cz.d2b : T(0);
+    z2c = gz1 ? cz.d2c : T(0);
+    mxy_a = gy2 ? cx.d1a : T(0);
+    mxy_b = gy2 ? cx.d1b : T(0);
+    mxz_a = gz2 ? cx.d1a : T(0);
+    mxz_b = gz2 ? cx.d1b : T(0);
+    myz_a = (gy2 && gz2) ? cy.d1a : T(0);
+    myz_b = (gy2 && gz2) ? cy.d1b : T(0);
+  }
+};
+
 // RK4 stage field at one site, formed on load from the field f, the velocity
 // v (ignored while the arrest holds, keep == 0) and the previous stage's
 // l = -dt g (core/integrator.py:382-396: Temp = F + f_s k_{s-1},

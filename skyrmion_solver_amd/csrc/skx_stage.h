@@ -137,51 +137,24 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   if ((A.flags & CHECK_DONE) && A.st->done) return;  // uniform per launch
 
   const int tid = threadIdx.x;
-  const int tz = tid % TZ, ty = tid / TZ;
-  // XCD-aware order: consecutive tiles (which share y halos) on the same L2
-  int lin = blk;
-  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
-  const int bz = lin % A.nbz;
-  lin /= A.nbz;
-  const int by = lin % A.nby;
-  const int bx = lin / A.nby;
-  const int z0 = bz * TZ, y0 = by * TY;
-  const int xs = bx * A.xchunk;
-  const int xe = min(xs + A.xchunk, A.X);
-  const int y = y0 + ty, z = z0 + tz;
+  const TileSite ts = tile_site<TZ, TY>(A, blk, nblk, tid);
+  const int tz = ts.tz, ty = ts.ty, z0 = ts.z0, y0 = ts.y0, xs = ts.xs, xe = ts.xe, y = ts.y, z = ts.z;
+  const bool own = ts.own;
   const int Y = A.Y, Z = A.Z, X = A.X, halo = A.halo;
-  const bool own = (y < Y) && (z < Z);
   const size_t N = (size_t)A.N;
   const size_t YZ = (size_t)Y * Z;
-
-  const bool gy1 = (y >= halo) && (y < Y - halo), gz1 = (z >= halo) && (z < Z - halo);
-  const bool gy2 = (y > halo) && (y < Y - halo), gz2 = (z > halo) && (z < Z - halo);
 
   const T dt = T(A.P.dt);
   T keep = T(1);
   if constexpr (STAGE >= 2) keep = T(A.st->keep);
   const StageCoef<T, STAGE> sc(dt, keep);
-  AxisCoef<T> cx, cy, cz;
-  cx.set(A.ilsx);
-  cy.set(A.ilsy);
-  cz.set(A.ilsz);
-  // per-thread guarded coefficients (the guards of core/derivatives.py:54,106,121 along y and z).
-  // The first differences along y and z are formed with their guards folded in, so they are 0 off
-  // the band at once; the mixed derivatives built from them carry stricter guards (i > halo), so an
-  // unguarded difference is never needed.
-  const T y1a = gy1 ? cy.d1a : T(0), y1b = gy1 ? cy.d1b : T(0);
-  const T z1a = gz1 ? cz.d1a : T(0), z1b = gz1 ? cz.d1b : T(0);
-  const T y2a = gy1 ? cy.d2a : T(0), y2b = gy1 ? cy.d2b : T(0), y2c = gy1 ? cy.d2c : T(0);
-  const T z2a = gz1 ? cz.d2a : T(0), z2b = gz1 ? cz.d2b : T(0), z2c = gz1 ? cz.d2c : T(0);
-  const T mxy_a = gy2 ? cx.d1a : T(0), mxy_b = gy2 ? cx.d1b : T(0);  // x-part of d_x d_y, y guard folded
-  const T mxz_a = gz2 ? cx.d1a : T(0), mxz_b = gz2 ? cx.d1b : T(0);
-  const T myz_a = (gy2 && gz2) ? cy.d1a : T(0), myz_b = (gy2 && gz2) ? cy.d1b : T(0);
+  const SiteCoef<T> gc(A, y, z);
+  const AxisCoef<T>& cx = gc.cx;
 
   // Tile elements this thread stages: element 0 is its own site (tile row
   // ty + 2, column tz + 2), so the raw values of its own column arrive in its
-  // own registers; elements 1.. are halo cells h = tid + (j - 1) NT, numbered
-  // rows 0, 1, TR-2, TR-1 in full, then columns 0, 1, TW-2, TW-1 of the rows
-  // between.  eidx = LDS index (-1: no element), eoff = byte offset in the
+  // own registers; elements 1.. are halo cells h = tid + (j - 1) NT (halo_cell
+  // numbering).  eidx = LDS index (-1: no element), eoff = byte offset in the
   // plane (kOOB outside the lattice: loads 0).
   int eidx[LPT];
   uint32_t eoff[LPT];
@@ -192,18 +165,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     if (j > 0) {
       const int h = tid + (j - 1) * NT;
       in_tile = h < NH;
-      if (h < 2 * TW) {
-        r = h / TW;
-        cc = h - r * TW;
-      } else if (h < 4 * TW) {
-        const int m = h - 2 * TW;
-        r = TR - 2 + m / TW;
-        cc = m % TW;
-      } else {
-        const int m = h - 4 * TW, q = m & 3;
-        r = 2 + m / 4;
-        cc = q < 2 ? q : TW - 4 + q;
-      }
+      halo_cell<TW, TR>(h, r, cc);
     }
     const int gy = y0 - 2 + r, gz = z0 - 2 + cc;
     const bool valid = in_tile && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
@@ -346,8 +308,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     for (int a = 0; a < NST; ++a) {
       auto t = [&](int e) { return fring[fi(fs, a, e)]; };
       qf[a][sp2] = t(C);
-      qy[a][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), y1a, y1b);
-      const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), z1a, z1b);
+      qy[a][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), gc.y1a, gc.y1b);
+      const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), gc.z1a, gc.z1b);
       qz[a][sp2] = dz;
       if constexpr (ANY_MIXED) {
         if ((Tr::MIXED >> a) & 1u) zring[zi(zs, a, (ty + 2) * TZ + tz)] = dz;
@@ -360,7 +322,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         for (int a = 0; a < NST; ++a) {
           if ((Tr::MIXED >> a) & 1u) {
             auto t = [&](int e) { return fring[fi(fs, a, e)]; };
-            zring[zi(zs, a, hrow * TZ + tz)] = cd1(t(h - 2), t(h - 1), t(h + 1), t(h + 2), z1a, z1b);
+            zring[zi(zs, a, hrow * TZ + tz)] = cd1(t(h - 2), t(h - 1), t(h + 1), t(h + 2), gc.z1a, gc.z1b);
           }
         }
       }
@@ -377,8 +339,8 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
       const bool gx1 = (c >= halo) && (c < X - halo), gx2 = (c > halo) && (c < X - halo);
       const T x1a = gx1 ? cx.d1a : T(0), x1b = gx1 ? cx.d1b : T(0);
       const T x2a = gx1 ? cx.d2a : T(0), x2b = gx1 ? cx.d2b : T(0), x2c = gx1 ? cx.d2c : T(0);
-      const T xya = gx2 ? mxy_a : T(0), xyb = gx2 ? mxy_b : T(0);
-      const T xza = gx2 ? mxz_a : T(0), xzb = gx2 ? mxz_b : T(0);
+      const T xya = gx2 ? gc.mxy_a : T(0), xyb = gx2 ? gc.mxy_b : T(0);
+      const T xza = gx2 ? gc.mxz_a : T(0), xzb = gx2 ? gc.mxz_b : T(0);
       const int cfs = (c + 8) & (FSLOTS - 1);
       const int czs = (c + 9) % ZSLOTS;
       T phi[NF], F[3][NST], S[6][NST];
@@ -392,14 +354,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         if constexpr (D2) {
           auto t = [&](int e) { return fring[fi(cfs, a, e)]; };
           S[0][a] = cd2(qf[a][sm2], qf[a][sm1], f0, qf[a][sp1], qf[a][sp2], x2a, x2b, x2c);
-          S[1][a] = cd2(t(C - 2 * TW), t(C - TW), f0, t(C + TW), t(C + 2 * TW), y2a, y2b, y2c);
-          S[2][a] = cd2(t(C - 2), t(C - 1), f0, t(C + 1), t(C + 2), z2a, z2b, z2c);
+          S[1][a] = cd2(t(C - 2 * TW), t(C - TW), f0, t(C + TW), t(C + 2 * TW), gc.y2a, gc.y2b, gc.y2c);
+          S[2][a] = cd2(t(C - 2), t(C - 1), f0, t(C + 1), t(C + 2), gc.z2a, gc.z2b, gc.z2c);
           if ((Tr::MIXED >> a) & 1u) {
             S[3][a] = cd1(qy[a][sm2], qy[a][sm1], qy[a][sp1], qy[a][sp2], xya, xyb);
             S[4][a] = cd1(qz[a][sm2], qz[a][sm1], qz[a][sp1], qz[a][sp2], xza, xzb);
             if constexpr (ANY_MIXED) {
               auto zr = [&](int row) { return zring[zi(czs, a, row * TZ + tz)]; };
-              S[5][a] = cd1(zr(ty), zr(ty + 1), zr(ty + 3), zr(ty + 4), myz_a, myz_b);
+              S[5][a] = cd1(zr(ty), zr(ty + 1), zr(ty + 3), zr(ty + 4), gc.myz_a, gc.myz_b);
             }
           } else {
             S[3][a] = S[4][a] = S[5][a] = T(0);

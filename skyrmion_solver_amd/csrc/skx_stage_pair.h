@@ -125,39 +125,18 @@ __device__ __forceinline__ void stage_body_pair(const StageArgs<T>& A, int blk, 
   const int hh = tid & 1;   // lane of the pair: components hh and hh + 2
   const int p0 = 2 * hh;    // its first position in a cell
   const int sid = tid >> 1; // site of the tile
-  const int tz = sid % TZ, ty = sid / TZ;
-  int lin = blk;
-  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
-  const int bz = lin % A.nbz;
-  lin /= A.nbz;
-  const int by = lin % A.nby;
-  const int bx = lin / A.nby;
-  const int z0 = bz * TZ, y0 = by * TY;
-  const int xs = bx * A.xchunk;
-  const int xe = min(xs + A.xchunk, A.X);
-  const int y = y0 + ty, z = z0 + tz;
+  const TileSite ts = tile_site<TZ, TY>(A, blk, nblk, sid);
+  const int tz = ts.tz, ty = ts.ty, z0 = ts.z0, y0 = ts.y0, xs = ts.xs, xe = ts.xe, y = ts.y, z = ts.z;
+  const bool own = ts.own;
   const int Y = A.Y, Z = A.Z, X = A.X, halo = A.halo;
-  const bool own = (y < Y) && (z < Z);
   const size_t N = (size_t)A.N;
   const size_t YZ = (size_t)Y * Z;
-
-  const bool gy1 = (y >= halo) && (y < Y - halo), gz1 = (z >= halo) && (z < Z - halo);
-  const bool gy2 = (y > halo) && (y < Y - halo), gz2 = (z > halo) && (z < Z - halo);
 
   const T dt = T(A.P.dt);
   const T keep = T(A.st->keep);
   const StageCoef<T, STAGE> sc(dt, keep);
-  AxisCoef<T> cx, cy, cz;
-  cx.set(A.ilsx);
-  cy.set(A.ilsy);
-  cz.set(A.ilsz);
-  const T y1a = gy1 ? cy.d1a : T(0), y1b = gy1 ? cy.d1b : T(0);
-  const T z1a = gz1 ? cz.d1a : T(0), z1b = gz1 ? cz.d1b : T(0);
-  const T y2a = gy1 ? cy.d2a : T(0), y2b = gy1 ? cy.d2b : T(0), y2c = gy1 ? cy.d2c : T(0);
-  const T z2a = gz1 ? cz.d2a : T(0), z2b = gz1 ? cz.d2b : T(0), z2c = gz1 ? cz.d2c : T(0);
-  const T mxy_a = gy2 ? cx.d1a : T(0), mxy_b = gy2 ? cx.d1b : T(0);
-  const T mxz_a = gz2 ? cx.d1a : T(0), mxz_b = gz2 ? cx.d1b : T(0);
-  const T myz_a = (gy2 && gz2) ? cy.d1a : T(0), myz_b = (gy2 && gz2) ? cy.d1b : T(0);
+  const SiteCoef<T> gc(A, y, z);
+  const AxisCoef<T>& cx = gc.cx;
 
   // Tile elements of the lane: element 0 is its site, elements 1.. the halo
   // cells sid + (j - 1) NS, numbered as in stage_body.  Offsets carry the
@@ -172,18 +151,7 @@ __device__ __forceinline__ void stage_body_pair(const StageArgs<T>& A, int blk, 
     if (j > 0) {
       const int hc = sid + (j - 1) * NS;
       in_tile = hc < NH;
-      if (hc < 2 * TW) {
-        r = hc / TW;
-        cc = hc - r * TW;
-      } else if (hc < 4 * TW) {
-        const int m = hc - 2 * TW;
-        r = TR - 2 + m / TW;
-        cc = m % TW;
-      } else {
-        const int m = hc - 4 * TW, q = m & 3;
-        r = 2 + m / 4;
-        cc = q < 2 ? q : TW - 4 + q;
-      }
+      halo_cell<TW, TR>(hc, r, cc);
     }
     const int gy = y0 - 2 + r, gz = z0 - 2 + cc;
     const bool valid = in_tile && gy >= 0 && gy < Y && gz >= 0 && gz < Z;
@@ -289,8 +257,8 @@ __device__ __forceinline__ void stage_body_pair(const StageArgs<T>& A, int blk, 
     for (int q = 0; q < 2; ++q) {
       auto t = [&](int e) { return fring[fi(fs, p0 + q, e)]; };
       qf[q][sp2] = t(C);
-      qy[q][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), y1a, y1b);
-      const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), z1a, z1b);
+      qy[q][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), gc.y1a, gc.y1b);
+      const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), gc.z1a, gc.z1b);
       qz[q][sp2] = dz;
       zring[zi(zs, p0 + q, (ty + 2) * TZ + tz)] = dz;
     }
@@ -299,7 +267,8 @@ __device__ __forceinline__ void stage_body_pair(const StageArgs<T>& A, int blk, 
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         auto t = [&](int e) { return fring[fi(fs, p0 + q, e)]; };
-        zring[zi(zs, p0 + q, hrow * TZ + tz)] = cd1(t(hcell - 2), t(hcell - 1), t(hcell + 1), t(hcell + 2), z1a, z1b);
+        zring[zi(zs, p0 + q, hrow * TZ + tz)] =
+            cd1(t(hcell - 2), t(hcell - 1), t(hcell + 1), t(hcell + 2), gc.z1a, gc.z1b);
       }
     }
 
@@ -308,8 +277,8 @@ __device__ __forceinline__ void stage_body_pair(const StageArgs<T>& A, int blk, 
       const bool gx1 = (c >= halo) && (c < X - halo), gx2 = (c > halo) && (c < X - halo);
       const T x1a = gx1 ? cx.d1a : T(0), x1b = gx1 ? cx.d1b : T(0);
       const T x2a = gx1 ? cx.d2a : T(0), x2b = gx1 ? cx.d2b : T(0), x2c = gx1 ? cx.d2c : T(0);
-      const T xya = gx2 ? mxy_a : T(0), xyb = gx2 ? mxy_b : T(0);
-      const T xza = gx2 ? mxz_a : T(0), xzb = gx2 ? mxz_b : T(0);
+      const T xya = gx2 ? gc.mxy_a : T(0), xyb = gx2 ? gc.mxy_b : T(0);
+      const T xza = gx2 ? gc.mxz_a : T(0), xzb = gx2 ? gc.mxz_b : T(0);
       const int cfs = (c + 8) & (FSLOTS - 1);
       const int czs = (c + 9) % ZSLOTS;
       T phi[2], F[3][2], S[6][2];
@@ -322,12 +291,12 @@ __device__ __forceinline__ void stage_body_pair(const StageArgs<T>& A, int blk, 
         F[2][q] = qz[q][s0];
         auto t = [&](int e) { return fring[fi(cfs, p0 + q, e)]; };
         S[0][q] = cd2(qf[q][sm2], qf[q][sm1], f0, qf[q][sp1], qf[q][sp2], x2a, x2b, x2c);
-        S[1][q] = cd2(t(C - 2 * TW), t(C - TW), f0, t(C + TW), t(C + 2 * TW), y2a, y2b, y2c);
-        S[2][q] = cd2(t(C - 2), t(C - 1), f0, t(C + 1), t(C + 2), z2a, z2b, z2c);
+        S[1][q] = cd2(t(C - 2 * TW), t(C - TW), f0, t(C + TW), t(C + 2 * TW), gc.y2a, gc.y2b, gc.y2c);
+        S[2][q] = cd2(t(C - 2), t(C - 1), f0, t(C + 1), t(C + 2), gc.z2a, gc.z2b, gc.z2c);
         S[3][q] = cd1(qy[q][sm2], qy[q][sm1], qy[q][sp1], qy[q][sp2], xya, xyb);
         S[4][q] = cd1(qz[q][sm2], qz[q][sm1], qz[q][sp1], qz[q][sp2], xza, xzb);
         auto zr = [&](int row) { return zring[zi(czs, p0 + q, row * TZ + tz)]; };
-        S[5][q] = cd1(zr(ty), zr(ty + 1), zr(ty + 3), zr(ty + 4), myz_a, myz_b);
+        S[5][q] = cd1(zr(ty), zr(ty + 1), zr(ty + 3), zr(ty + 4), gc.myz_a, gc.myz_b);
       }
       const size_t so = (size_t)c * YZ + own_off;
       T g[2];
