@@ -254,6 +254,12 @@ int skx_history_read(skx_handle h, skx_step_record* out, int32_t n, int32_t* n_r
 int skx_trace_read(skx_handle h, double ms[5], int64_t counts[5]);
 int skx_trace_reset(skx_handle h);
 
+/* Launch shape the engine chose for stage kind 0..4 (no counterpart in the
+ * reference; for the autotuner and diagnostics): shape[0] x shape[1] = the
+ * (z, y) tile in sites, shape[2] = planes per block along x, shape[3] =
+ * threads per site. */
+int skx_stage_shape(skx_handle h, int32_t stage, int32_t shape[4]);
+
 /* Wait for all work queued on the handle's stream. */
 int skx_sync(skx_handle h);
 

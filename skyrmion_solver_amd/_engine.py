@@ -43,7 +43,7 @@ EXPORTED = (
     "skx_create", "skx_destroy", "skx_footprint_bytes", "skx_upload", "skx_download", "skx_anf_step",
     "skx_anf_run", "skx_anf_prepare", "skx_rk4_steps", "skx_energy", "skx_baryon", "skx_density",
     "skx_eval_gradient",
-    "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_sync",
+    "skx_history_attach", "skx_history_read", "skx_trace_read", "skx_trace_reset", "skx_stage_shape", "skx_sync",
     "skx_batch_create", "skx_batch_destroy", "skx_batch_add", "skx_batch_run", "skx_batch_launch",
     "skx_batch_wait", "skx_batch_poll",
     "skx_batch_remove", "skx_diag_stream_copy", "skx_device_info", "skx_last_error", "skx_version",
@@ -146,6 +146,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "skx_history_read": ([H, C.POINTER(StepRecord), C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "skx_trace_read": ([H, dp, C.POINTER(C.c_int64)], C.c_int),
         "skx_trace_reset": ([H], C.c_int),
+        "skx_stage_shape": ([H, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "skx_sync": ([H], C.c_int),
         "skx_batch_create": ([C.POINTER(BatchDesc), C.POINTER(H)], C.c_int),
         "skx_batch_destroy": ([H], C.c_int),
@@ -339,6 +340,12 @@ class Engine:
 
     def trace_reset(self):
         _check(self._lib.skx_trace_reset(self._h), "skx_trace_reset")
+
+    def stage_shape(self, stage: int) -> dict:
+        """Launch shape of stage kind `stage` (0 = energy pass, 1..4 = RK4 stages)."""
+        buf = (C.c_int32 * 4)()
+        _check(self._lib.skx_stage_shape(self._h, int(stage), buf), "skx_stage_shape")
+        return dict(tile_z=buf[0], tile_y=buf[1], x_chunk=buf[2], lanes=buf[3])
 
     def sync(self):
         _check(self._lib.skx_sync(self._h), "skx_sync")

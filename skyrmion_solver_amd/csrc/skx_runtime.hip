@@ -124,30 +124,45 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
 }
 
 // Default tile of one stage.  In FP32 some stages run on a 32x16 tile, chosen
-// per theory and stage from measurement at 128^3 (wide_tile_stage).  Where a
-// kernel holds 2 waves per SIMD on either tile, the taller tile cuts the
+// per theory and stage from measurement (wide_tile_stage).  Where a kernel
+// holds the same waves per SIMD on either tile, the taller tile cuts the
 // y-halo share of every plane from 1.69 to 1.41 loads per site (pion: stage 1
-// 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us); pion stages 2 and 3 keep 32x8
-// (32x16 measured no faster for them).  A forced
-// tile_z and small lattices keep the base tile for every stage; a batch makes
-// the same choice for all its lattices at creation (skx_batch_create).
+// 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us at 128^3); pion stages 2 and 3 keep
+// 32x8 (32x16 measured no faster for them).  A forced tile_z and small
+// lattices keep the base tile for every stage; a batch makes the same choice
+// for all its lattices at creation (skx_batch_create), as for a small lattice.
 //
-// Coulomb's stages 1 and 4 (5 components) and Berger's stage 1 need 256 VGPRs
-// and run 1 wave per SIMD on 32x8, while a 32x16 block holds 2 with some
-// spilling (128^3 FP32, us): Coulomb stage 1 80 -> 70, stage 4 111 -> 88;
-// Berger stage 1 75 -> 56; lightly bound stage 1 54 -> 49.  Omega and rho
-// keep 32x8: their 32x16 gains at 128^3 were 2-3%, and in a batch of a few
-// small lattices the taller tile halves the blocks, which cost the serving
-// sample 8%.
-bool wide_tile_stage(int theory, int stage) {
-  if (theory == SKX_THEORY_OMEGA || theory == SKX_THEORY_RHO) return false;
-  return stage <= 1 || stage == 4;
+// Berger's stage 1 needs 256 VGPRs and runs 1 wave per SIMD on 32x8, while a
+// 32x16 block holds 2 (128^3 FP32: 75 -> 56 us; lightly bound stage 1 54 -> 49).
+//
+// The five-component theories depend on the lattice's size.  Beyond the
+// 256 MiB Infinity Cache (the seven FP32 field arrays of a 160^3 lattice take
+// 573 MB) the 32x8 tile's extra y-halo rows are fetched from HBM, and the
+// taller tile wins (160^3 B=8, us): omega stage 1 73.9 -> 69.3, stage 4
+// 211.5 -> 177.9 (+8% per step); Coulomb stage 2 125.4 -> 105.3, stage 3
+// 172.8 -> 140.8 (+11%).  Within it the two tiles are within 1-4% per stage at
+// 128^3, and in batches of 64^3-96^3 omega lattices the 32x16 stages 1 and 4
+// ran 11-15% slower (half the blocks to spread over the CUs).  Coulomb's stage
+// 1 (248 VGPRs, 2 waves per SIMD on 32x8) is 5% faster on 32x8 at both sizes.
+// Rho keeps 32x8: its stage 1 ran 218 -> 283 us on 32x16 at 128^3.
+bool wide_tile_stage(int theory, int stage, bool beyond_cache) {
+  switch (theory) {
+    case SKX_THEORY_RHO: return false;
+    case SKX_THEORY_OMEGA: return beyond_cache && (stage == 1 || stage == 4);
+    case SKX_THEORY_COULOMB: return stage == 0 || stage == 4 || (beyond_cache && (stage == 2 || stage == 3));
+    default: return stage <= 1 || stage == 4;
+  }
+}
+// the FP32 form's seven field arrays (F, V ping-pong pairs and l1..l3) exceed
+// the 256 MiB Infinity Cache
+bool beyond_infinity_cache(const Lattice& L) {
+  return 7.0 * L.nf * (double)L.N * sizeof(float) > 256.0 * 1024 * 1024;
 }
 constexpr LaunchShape kWideTile{32, 16, 1};
 
 LaunchShape stage_tile(const Lattice& L, const LaunchShape& base, int stage, bool per_stage) {
   if (!per_stage || L.prec != SKX_FP32 || base.tz != 32 || base.ty != 8 || L.Y < 32) return base;
-  return wide_tile_stage(L.theory, stage) ? kWideTile : base;
+  return wide_tile_stage(L.theory, stage, beyond_infinity_cache(L)) ? kWideTile : base;
 }
 
 // Planes per block along x for one stage kernel: as many x-chunks as keep
@@ -983,6 +998,17 @@ int skx_trace_read(skx_handle h, double ms[5], int64_t counts[5]) {
   return SKX_OK;
 }
 
+int skx_stage_shape(skx_handle h, int32_t stage, int32_t shape[4]) {
+  CHECK_HANDLE(h);
+  if (stage < 0 || stage > 4 || !shape) return fail(SKX_EINVAL, "stage must be 0..4 and shape non-null");
+  const LaunchShape& s = h->shape[stage];
+  shape[0] = s.tz;
+  shape[1] = s.ty;
+  shape[2] = s.xchunk;
+  shape[3] = s.lanes;
+  return SKX_OK;
+}
+
 int skx_trace_reset(skx_handle h) {
   CHECK_HANDLE(h);
   HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1172,7 +1198,7 @@ int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   b->tile = d->tile_z == 64 ? LaunchShape{64, 4, 1} : (d->tile_z == 16 ? LaunchShape{16, 16, 1} : LaunchShape{32, 8, 1});
   // tile_z 0: the FP32 per-stage defaults of a single lattice (stage_tile), for every lattice of the batch
   for (int stage = 1; stage <= 4; ++stage) {
-    const bool wide = d->tile_z == 0 && d->precision == SKX_FP32 && wide_tile_stage(d->theory, stage);
+    const bool wide = d->tile_z == 0 && d->precision == SKX_FP32 && wide_tile_stage(d->theory, stage, false);
     b->stile[stage] = wide ? kWideTile : b->tile;
     if (stage >= 2 && pair_theory(d->theory, d->precision) && !(d->flags & SKX_FLAG_ONE_LANE))
       b->stile[stage].lanes = 2;

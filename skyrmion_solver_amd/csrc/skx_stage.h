@@ -202,7 +202,12 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
       }
   };
 
-  // own-column x-queues, 5 slots rotated by the unrolled plane loop
+  // own-column x-queues, 5 slots rotated by the unrolled plane loop: the stage
+  // field of every stencil component, and the raw y / z first differences of
+  // the components with mixed derivatives (the others form theirs at the
+  // centre from the LDS plane, sharing the taps of their second differences,
+  // so that no queue holds them: omega's five components need no mixed
+  // derivative, and this keeps its stage kernels at two waves per SIMD)
   T qf[NST][5], qy[NST][5], qz[NST][5];
 #pragma unroll
   for (int a = 0; a < NST; ++a)
@@ -302,17 +307,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     __syncthreads();
     if constexpr (decltype(NEXT)::value) issue(pre, p + 1);  // past the end of the chunk: never read
 
-    // lead plane p: own-column value and raw first differences along y and z
+    // lead plane p: own-column value and, for mixed components, raw first differences along y and z
     const int zs = (p + 9) % ZSLOTS;
 #pragma unroll
     for (int a = 0; a < NST; ++a) {
       auto t = [&](int e) { return fring[fi(fs, a, e)]; };
       qf[a][sp2] = t(C);
-      qy[a][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), gc.y1a, gc.y1b);
-      const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), gc.z1a, gc.z1b);
-      qz[a][sp2] = dz;
-      if constexpr (ANY_MIXED) {
-        if ((Tr::MIXED >> a) & 1u) zring[zi(zs, a, (ty + 2) * TZ + tz)] = dz;
+      if ((Tr::MIXED >> a) & 1u) {
+        qy[a][sp2] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), gc.y1a, gc.y1b);
+        const T dz = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), gc.z1a, gc.z1b);
+        qz[a][sp2] = dz;
+        if constexpr (ANY_MIXED) zring[zi(zs, a, (ty + 2) * TZ + tz)] = dz;
       }
     }
     if constexpr (ANY_MIXED) {
@@ -349,8 +354,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
         const T f0 = qf[a][s0];
         phi[a] = f0;
         F[0][a] = cd1(qf[a][sm2], qf[a][sm1], qf[a][sp1], qf[a][sp2], x1a, x1b);
-        F[1][a] = qy[a][s0];
-        F[2][a] = qz[a][s0];
+        if ((Tr::MIXED >> a) & 1u) {
+          F[1][a] = qy[a][s0];
+          F[2][a] = qz[a][s0];
+        } else {
+          auto t = [&](int e) { return fring[fi(cfs, a, e)]; };
+          F[1][a] = cd1(t(C - 2 * TW), t(C - TW), t(C + TW), t(C + 2 * TW), gc.y1a, gc.y1b);
+          F[2][a] = cd1(t(C - 2), t(C - 1), t(C + 1), t(C + 2), gc.z1a, gc.z1b);
+        }
         if constexpr (D2) {
           auto t = [&](int e) { return fring[fi(cfs, a, e)]; };
           S[0][a] = cd2(qf[a][sm2], qf[a][sm1], f0, qf[a][sp1], qf[a][sp2], x2a, x2b, x2c);
@@ -591,7 +602,7 @@ hipError_t visit_shape(int stage, const LaunchShape& s, Fn&& fn) {
   if (s.tz == 32 && s.ty == 8) return visit_stage<T, TH, 32, 8>(stage, fn);
   if (s.tz == 16 && s.ty == 16) return visit_stage<T, TH, 16, 16>(stage, fn);
   // 32x16: the FP32 default of some stages (the runtime's stage_tile)
-  if constexpr (sizeof(T) == 4 && TH != OMEGA && TH != RHO) {
+  if constexpr (sizeof(T) == 4 && TH != RHO) {
     if (s.tz == 32 && s.ty == 16) return visit_stage<T, TH, 32, 16>(stage, fn);
   }
   return hipErrorInvalidValue;
