@@ -42,8 +42,7 @@ __device__ __forceinline__ void rho_curl_body(const StageArgs<T>& A, int blk, in
 
   const int tid = threadIdx.x;
   const int tz = tid % TZ, ty = tid / TZ;
-  int lin = blk;
-  if ((nblk & 7) == 0) lin = (lin & 7) * (nblk >> 3) + (lin >> 3);
+  int lin = xcd_tile_order(blk, nblk);
   const int per_iso = A.nbz * A.nby * A.nbx;
   const int iso = lin / per_iso;
   lin -= iso * per_iso;
