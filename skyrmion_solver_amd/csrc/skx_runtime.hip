@@ -124,45 +124,42 @@ LaunchShape choose_tile(const Lattice& L, const skx_desc* d) {
 }
 
 // Default tile of one stage.  In FP32 some stages run on a 32x16 tile, chosen
-// per theory and stage from measurement (wide_tile_stage).  Where a kernel
-// holds the same waves per SIMD on either tile, the taller tile cuts the
-// y-halo share of every plane from 1.69 to 1.41 loads per site (pion: stage 1
-// 53.8 -> 48.3 us, stage 4 74.1 -> 68.8 us at 128^3); pion stages 2 and 3 keep
-// 32x8 (32x16 measured no faster for them).  A forced tile_z and small
-// lattices keep the base tile for every stage; a batch makes the same choice
-// for all its lattices at creation (skx_batch_create), as for a small lattice.
+// per theory and stage from measurement (wide_tile_stage).  The taller tile
+// cuts the y-halo share of every plane from 1.69 to 1.41 loads per site, but
+// has half the blocks to spread over the CUs.  A forced tile_z and lattices
+// with Y < 32 keep the base tile for every stage; a batch makes its choice for
+// all its lattices at creation (skx_batch_create).
 //
-// Berger's stage 1 needs 256 VGPRs and runs 1 wave per SIMD on 32x8, while a
-// 32x16 block holds 2 (128^3 FP32: 75 -> 56 us; lightly bound stage 1 54 -> 49).
+// Pion: stage 4 (FP32 B=4, us) 56.7 -> 53.7 at 128^3 and 133.2 -> 125.9 at
+// 160^3 on 32x16; stages 0-3 are 1-5% faster on 32x8 (stage 1: 37.5 against
+// 38.6, and 62.0 against 64.4).  Berger's stage 1 needs 256 VGPRs and runs 1
+// wave per SIMD on 32x8, while a 32x16 block holds 2 (128^3: 75 -> 56 us;
+// lightly bound stage 1 54 -> 49), so both run stages 0, 1 and 4 on 32x16.
 //
-// The five-component theories depend on the lattice's size.  Beyond the
-// 256 MiB Infinity Cache (the seven FP32 field arrays of a 160^3 lattice take
-// 573 MB) the 32x8 tile's extra y-halo rows are fetched from HBM, and the
-// taller tile wins (160^3 B=8, us): omega stage 1 73.9 -> 69.3, stage 4
-// 211.5 -> 177.9 (+8% per step); Coulomb stage 2 125.4 -> 105.3, stage 3
-// 172.8 -> 140.8 (+11%).  Within it the two tiles are within 1-4% per stage at
-// 128^3, and in batches of 64^3-96^3 omega lattices the 32x16 stages 1 and 4
-// ran 11-15% slower (half the blocks to spread over the CUs).  Coulomb's stage
-// 1 (248 VGPRs, 2 waves per SIMD on 32x8) is 5% faster on 32x8 at both sizes.
-// Rho keeps 32x8: its stage 1 ran 218 -> 283 us on 32x16 at 128^3.
-bool wide_tile_stage(int theory, int stage, bool beyond_cache) {
+// Omega and Coulomb (five components): their stage 4 holds 1 wave per SIMD
+// on 32x8 (Coulomb: 242 VGPRs and 88 KB of LDS per 4-wave block) and 2 on
+// 32x16, and runs faster there at both sizes measured (FP32, B=8, us):
+// omega 67.7 -> 62.1 at 128^3, 161.8 -> 151.4 at 160^3; Coulomb 100.6 -> 73.5
+// and 258.3 -> 176.7.  Their other stages are as fast or faster on 32x8
+// (Coulomb stages 2 and 3 at 160^3: 89.6 and 108.9 against 99.3 and 125.7).
+// In a batch, omega keeps 32x8 for every stage: in batches of 64^3-96^3
+// lattices its 32x16 stages 1 and 4 ran 11-15% slower (half the blocks to
+// spread over the CUs).  Rho keeps 32x8: its stage 1 ran 218 -> 283 us on
+// 32x16 at 128^3.
+bool wide_tile_stage(int theory, int stage, bool batch) {
   switch (theory) {
     case SKX_THEORY_RHO: return false;
-    case SKX_THEORY_OMEGA: return beyond_cache && (stage == 1 || stage == 4);
-    case SKX_THEORY_COULOMB: return stage == 0 || stage == 4 || (beyond_cache && (stage == 2 || stage == 3));
+    case SKX_THEORY_OMEGA: return !batch && stage == 4;
+    case SKX_THEORY_COULOMB:
+    case SKX_THEORY_PION: return stage == 4;
     default: return stage <= 1 || stage == 4;
   }
-}
-// the FP32 form's seven field arrays (F, V ping-pong pairs and l1..l3) exceed
-// the 256 MiB Infinity Cache
-bool beyond_infinity_cache(const Lattice& L) {
-  return 7.0 * L.nf * (double)L.N * sizeof(float) > 256.0 * 1024 * 1024;
 }
 constexpr LaunchShape kWideTile{32, 16, 1};
 
 LaunchShape stage_tile(const Lattice& L, const LaunchShape& base, int stage, bool per_stage) {
   if (!per_stage || L.prec != SKX_FP32 || base.tz != 32 || base.ty != 8 || L.Y < 32) return base;
-  return wide_tile_stage(L.theory, stage, beyond_infinity_cache(L)) ? kWideTile : base;
+  return wide_tile_stage(L.theory, stage, false) ? kWideTile : base;
 }
 
 // Planes per block along x for one stage kernel: as many x-chunks as keep
@@ -1198,7 +1195,7 @@ int skx_batch_create(const skx_batch_desc* d, skx_batch_handle* out) {
   b->tile = d->tile_z == 64 ? LaunchShape{64, 4, 1} : (d->tile_z == 16 ? LaunchShape{16, 16, 1} : LaunchShape{32, 8, 1});
   // tile_z 0: the FP32 per-stage defaults of a single lattice (stage_tile), for every lattice of the batch
   for (int stage = 1; stage <= 4; ++stage) {
-    const bool wide = d->tile_z == 0 && d->precision == SKX_FP32 && wide_tile_stage(d->theory, stage, false);
+    const bool wide = d->tile_z == 0 && d->precision == SKX_FP32 && wide_tile_stage(d->theory, stage, true);
     b->stile[stage] = wide ? kWideTile : b->tile;
     if (stage >= 2 && pair_theory(d->theory, d->precision) && !(d->flags & SKX_FLAG_ONE_LANE))
       b->stile[stage].lanes = 2;

@@ -209,16 +209,23 @@ def test_fp32_form_against_fp64_oracle(theory):
     assert rel_l2(Fg, st.F) <= 1e-5
 
 
-@pytest.mark.parametrize("theory", ["pion", "berger", "lightly", "coulomb"])
+WIDE_STAGES = {"pion": {4}, "omega": {4}, "coulomb": {4}, "berger": {0, 1, 4}, "lightly": {0, 1, 4}}
+
+
+@pytest.mark.parametrize("theory", ["pion", "omega", "coulomb", "berger", "lightly"])
 def test_fp32_default_mixed_tiles_equal_uniform_tile(theory):
     """FP32 lattices with Y >= 32 run some stages on a 32x16 tile and the others on 32x8 (the runtime's
-    stage_tile); the per-site arithmetic does not depend on the tile, so the fields after the same steps
-    are bit-identical to an all-32x8 run (tile_z=32) and the energies agree to reduction order."""
+    stage_tile, reported by skx_stage_shape); the per-site arithmetic does not depend on the tile, so the
+    gradient and the fields after the same steps are bit-identical to an all-32x8 run (tile_z=32) and
+    the energies agree to reduction order."""
     mod, rp, p_i, p_f = make_params(theory, 22, 40, 36)
     F = smooth_field(rp, int(p_i[5]), seed=8, normalise=True)
     st = OI.OracleState(theory, p_i, p_f, F)
     with _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32) as a, \
             _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32, tile_z=32) as b:
+        for k in range(5):
+            assert a.stage_shape(k)["tile_y"] == (16 if k in WIDE_STAGES[theory] else 8), k
+            assert b.stage_shape(k)["tile_y"] == 8, k
         a.upload(_engine.FIELD, F)
         b.upload(_engine.FIELD, F)
         ea, eb = a.energy(), b.energy()
@@ -235,30 +242,3 @@ def test_fp32_default_mixed_tiles_equal_uniform_tile(theory):
         assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
         assert np.array_equal(a.download(_engine.VELOCITY), b.download(_engine.VELOCITY))
 
-
-@pytest.mark.parametrize("theory", ["omega", "coulomb"])
-def test_fp32_tiles_beyond_infinity_cache_equal_uniform_tile(theory):
-    """A 136^3 five-component FP32 lattice (seven field arrays of 352 MB, beyond the 256 MiB Infinity
-    Cache) runs omega's stages 1 and 4 and Coulomb's stages 0, 2, 3 and 4 on the 32x16 tile, which
-    smaller lattices do not: the gradient, and the fields and velocities after the same steps, are
-    bit-identical to an all-32x8 run (tile_z=32), and the energies agree to reduction order."""
-    mod, rp, p_i, p_f = make_params(theory, 136, 136, 136)
-    F = smooth_field(rp, int(p_i[5]), seed=12, normalise=True)
-    with _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32) as a, \
-            _engine_for(theory, rp, p_i, p_f, prec=_engine.FP32, tile_z=32) as b:
-        wide = {"omega": {1, 4}, "coulomb": {0, 2, 3, 4}}[theory]
-        for k in range(5):
-            assert a.stage_shape(k)["tile_y"] == (16 if k in wide else 8), k
-            assert b.stage_shape(k)["tile_y"] == 8, k
-        a.upload(_engine.FIELD, F)
-        b.upload(_engine.FIELD, F)
-        ga, _ = a.eval_gradient(F)
-        gb, _ = b.eval_gradient(F)
-        assert np.array_equal(ga, gb)
-        sa = a.anf_run(12, -1.0, 4)
-        sb = b.anf_run(12, -1.0, 4)
-        assert sa.steps == sb.steps == 12
-        assert sa.err_signed == sb.err_signed and sa.err_abs == sb.err_abs
-        assert abs(sa.energy - sb.energy) <= 1e-13 * abs(sb.energy)
-        assert np.array_equal(a.download(_engine.FIELD), b.download(_engine.FIELD))
-        assert np.array_equal(a.download(_engine.VELOCITY), b.download(_engine.VELOCITY))
