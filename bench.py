@@ -327,7 +327,11 @@ def main():
                                                                "(0: skip)")
     ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
                     help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
+    ap.add_argument("--size", type=int, default=128,
+                    help="profiling aid with --only fp32/fp64: lattice edge of the pion B=4 workload")
     args = ap.parse_args()
+    if args.size != 128 and args.only not in ("fp32", "fp64"):
+        ap.error("--size applies to --only fp32 / fp64 runs; the metric is quoted at 128^3")
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
         sys.exit(2)
@@ -336,7 +340,7 @@ def main():
         gbs, ms = _engine.stream_copy_bandwidth(1 << 30, 20)
         print(f"[bench] stream copy of 1 GiB: {gbs:.1f} GB/s, {ms:.4f} ms per copy", file=sys.stderr, flush=True)
         return
-    rp, p_i, p_f, F = workload()
+    rp, p_i, p_f, F = workload(L=args.size)
     N = int(p_i[6])
     nf = 4
     if args.only in ("fp32", "fp64"):

@@ -384,6 +384,9 @@ int build_graph(skx_engine* e, int parity, int steps, int fuse_baryon) {
   (void)hipGraphDestroy(g);
   if (r3 != hipSuccess) return fail(SKX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r3));
   e->graph[parity] = ex;
+  // upload the executable graph now, so that its first launch (inside a run,
+  // perhaps a timed one) does not pay for the upload of its kernel nodes
+  HIP_TRY(hipGraphUpload(ex, e->stream));
   return SKX_OK;
 }
 
@@ -1136,6 +1139,7 @@ int batch_graph(skx_batch* b, int steps) {
     return fail(SKX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r3));
   }
   b->graph_steps = steps;
+  HIP_TRY(hipGraphUpload(b->graph, b->stream));
   return SKX_OK;
 }
 
