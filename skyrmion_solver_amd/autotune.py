@@ -116,6 +116,19 @@ def chunk_rule(L: int, target: int) -> int:
     return -(-L // n)
 
 
+def chunk_for_blocks(X: int, Y: int, Z: int, target_blocks: int, tile_z: int = 0, min_planes: int = 8) -> int:
+    """x-chunk that gives an X x Y x Z batch lattice about `target_blocks` blocks of the (tile_z x 256/tile_z)
+    tile (tile_z 0: the 32 x 8 base tile), with at least `min_planes` planes per block (0 = the whole
+    lattice).  Small lattices get short chunks, so that a batch of a few of them still spreads over the
+    CUs; large ones get long chunks, which re-read fewer x-halo planes."""
+    tz = tile_z or 32
+    ty = 256 // tz
+    cols = max(1, -(-Z // tz) * -(-Y // ty))
+    n = max(1, round(target_blocks / cols))
+    xc = max(min_planes, -(-X // n))
+    return 0 if xc >= X else xc
+
+
 def batch_blocks(dims, tile_z: int, target: int) -> int:
     """Blocks of a batch of lattices `dims` (X, Y, Z) with the (tile_z x 256/tile_z) tile and the
     ``chunk_rule(X, target)`` x-chunks.  tile_z 0 (a tile per stage) counts its base 32 x 8 tile,

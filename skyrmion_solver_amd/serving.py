@@ -79,7 +79,7 @@ class Server:
                  max_slots: int = 64, max_blocks: int = 32768, tile_z: int = 0, device: int = 0,
                  keep_fields: bool = True, batch_factory=None, footprint_fn=None, clock=time.perf_counter,
                  refine_fp64: bool = False, refine_factor: float = 3.0, refine_batch_factory=None,
-                 refine_footprint_fn=None, x_chunk_target: int = 32):
+                 refine_footprint_fn=None, x_chunk_target: int = 32, x_chunk_blocks: int = 256):
         self.precision = precision
         self.budget = int(hbm_budget_bytes)
         self.chunk = int(chunk_steps)
@@ -89,10 +89,15 @@ class Server:
             THEORY_IDS[theory], precision, max_slots=max_slots, max_blocks=max_blocks, tile_z=tile_z, device=device))
         self._footprint = footprint_fn or (lambda r: _engine.footprint_bytes(
             THEORY_IDS[r.theory], precision, r.p_i, r.p_f, tile_z=tile_z))
-        # planes per block along x for each admitted lattice (``autotune.chunk_rule``; 0: the whole
-        # lattice): shorter chunks give a batch of a few small lattices enough blocks to fill the chip
-        # (bench serving sample, FP32: p50 15.5 s with whole lattices, 14.2 s with 32, 15.2 s with 16)
+        # planes per block along x for each admitted lattice: about `x_chunk_blocks` blocks per lattice
+        # with at least 8 planes per block (``autotune.chunk_for_blocks``), or, with x_chunk_blocks 0,
+        # about `x_chunk_target` planes per block (``autotune.chunk_rule``; 0: the whole lattice).  A
+        # few small lattices (the omega requests at 64^3 that run 50k steps) then still spread over the
+        # CUs: on the bench's serving sample (8 requests, rate 20/s, FP32 refined in FP64) the makespan
+        # was 21.5 s with 32 planes per block, 14.5 s with 16, 11.8 s with 8 and 11.4 s with 256 blocks
+        # per lattice, with the same steps and results per request
         self.x_chunk_target = int(x_chunk_target)
+        self.x_chunk_blocks = int(x_chunk_blocks)
         self.refine = bool(refine_fp64) and precision == _engine.FP32
         self.refine_factor = float(refine_factor)
         self._refine_factory = refine_batch_factory or (lambda theory: _engine.Batch(
@@ -117,10 +122,14 @@ class Server:
     def submit(self, req: Request):
         self.queue.append(req)
 
-    def _x_chunk(self, req) -> int:
-        from skyrmion_solver_amd.autotune import chunk_rule
+    def x_chunk(self, req) -> int:
+        """Planes per block along x with which a request's lattice is admitted (0: the whole lattice)."""
+        from skyrmion_solver_amd.autotune import chunk_for_blocks, chunk_rule
 
-        return chunk_rule(int(req.p_i[0]), self.x_chunk_target)
+        X, Y, Z = (int(v) for v in req.p_i[:3])
+        if self.x_chunk_blocks > 0:
+            return chunk_for_blocks(X, Y, Z, self.x_chunk_blocks, self.tile_z)
+        return chunk_rule(X, self.x_chunk_target)
 
     def _batch(self, theory: str, kind: int = 0):
         if (kind, theory) not in self.batches:
@@ -137,7 +146,7 @@ class Server:
                 return False
             try:
                 slot = self._batch(req.theory, 1).add(req.p_i, req.p_f, F, V, max_steps=left, tol=req.tol,
-                                                      device=self.device, x_chunk=self._x_chunk(req))
+                                                      device=self.device, x_chunk=self.x_chunk(req))
             except _engine.SkxError as e:
                 if e.code == -5:
                     return False
@@ -161,7 +170,7 @@ class Server:
             try:
                 slot = self._batch(req.theory).add(req.p_i, req.p_f, req.F0, req.V0, max_steps=req.max_steps,
                                                    tol=req.tol, device=self.device, init=req.init,
-                                                   x_chunk=self._x_chunk(req))
+                                                   x_chunk=self.x_chunk(req))
             except _engine.SkxError as e:
                 if e.code == -5:  # SKX_EBUSY: no slot / blocks free in this batch now
                     return

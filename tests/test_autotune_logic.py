@@ -24,6 +24,27 @@ def test_chunk_rule_splits_evenly_and_never_exceeds_target_by_much():
                 assert c <= 2 * tgt
 
 
+def test_chunk_for_blocks_targets_the_block_count():
+    # 64^3, 32x8 tiles: 2 x 8 = 16 columns; 256 blocks would need 4-plane chunks, the floor is 8
+    assert autotune.chunk_for_blocks(64, 64, 64, 256) == 8
+    # 128^3: 64 columns -> 4 chunks of 32
+    assert autotune.chunk_for_blocks(128, 128, 128, 256) == 32
+    # 192^3: 144 columns -> 2 chunks of 96
+    assert autotune.chunk_for_blocks(192, 192, 192, 256) == 96
+    # 16x16 tiles: 64^3 has 16 columns too
+    assert autotune.chunk_for_blocks(64, 64, 64, 256, tile_z=16) == 8
+    # a lattice whose columns alone reach the target runs whole
+    assert autotune.chunk_for_blocks(64, 512, 512, 256) == 0
+    for L in range(8, 260, 7):
+        for tgt in (64, 128, 256, 512):
+            c = autotune.chunk_for_blocks(L, L, L, tgt)
+            if c:
+                assert 8 <= c < L
+                cols = -(-L // 32) * -(-L // 8)
+                nb = cols * -(-L // c)
+                assert nb <= max(2 * tgt, cols * -(-L // 8))
+
+
 def test_batch_blocks_counts_tiles_and_chunks():
     # 128^3 with 32x8 tiles: 4 z-tiles x 16 y-tiles; 4 x-chunks of 32
     assert autotune.batch_blocks([(128, 128, 128)], 32, 32) == 4 * 16 * 4

@@ -169,8 +169,8 @@ def test_c4_batch_of_48_equals_solo_bitwise():
 def test_c5_reduced_stream_equals_solo(prec):
     """Nine requests of the serving stream's theory mix (pion / omega / rho) at 64^3..192^3 with staggered
     arrivals and different step budgets, served by continuous batching with the server's defaults (tile
-    per stage, ~32-plane x-chunks, fields built on the device at admission); each result equals the same
-    request run alone with the same launch shape, bit for bit."""
+    per stage, about 256 blocks per lattice, fields built on the device at admission); each result equals
+    the same request run alone with the same x-chunk, bit for bit."""
     print(f"[case] c5 reduced stream 64^3..192^3 prec={prec}", flush=True)
     reqs = workloads.c5_requests(n=9, rate=50.0, seed=3, sizes=(64, 96, 128, 160, 192), device_init=True)
     for k, r in enumerate(reqs):
@@ -182,7 +182,7 @@ def test_c5_reduced_stream_equals_solo(prec):
     assert rep["requests"] == len(reqs)
     for r in reqs:
         L = int(r.p_i[0])
-        xc = autotune.chunk_rule(L, 32) or L
+        xc = srv.x_chunk(r) or L
         with _engine.Engine(THEORY_IDS[r.theory], prec, r.p_i, r.p_f, x_chunk=xc) as e:
             e.init_field(r.init)
             st = e.anf_run(r.max_steps, r.tol, 8)

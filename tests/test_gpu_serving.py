@@ -35,7 +35,7 @@ def test_served_results_equal_solo(prec):
         reqs.append(Request(rid=k, theory=th, p_i=p_i, p_f=p_f, F0=F, tol=2.0, max_steps=120 + 10 * k,
                             arrival=0.003 * k))
     srv = Server(precision=prec, hbm_budget_bytes=1 << 30, chunk_steps=8, max_slots=8, max_blocks=2048,
-                 tile_z=32, x_chunk_target=0)  # the solo runs below use the same tile and chunk: bit for bit
+                 tile_z=32, x_chunk_target=0, x_chunk_blocks=0)  # the solo runs below: same tile and chunk, bit for bit
     rep = srv.serve(reqs)
     srv.close()
     print("  report", rep, flush=True)
@@ -61,7 +61,7 @@ def test_fp32_refinement_equals_the_split_run_by_hand():
         p_i, p_f, F = build(th, sh, 60 + k, _engine.FP32)
         reqs.append(Request(rid=k, theory=th, p_i=p_i, p_f=p_f, F0=F, tol=tol, max_steps=160, arrival=0.002 * k))
     srv = Server(precision=_engine.FP32, hbm_budget_bytes=1 << 30, chunk_steps=chunk, max_slots=4, max_blocks=512,
-                 tile_z=32, x_chunk_target=0, refine_fp64=True, refine_factor=factor)
+                 tile_z=32, x_chunk_target=0, x_chunk_blocks=0, refine_fp64=True, refine_factor=factor)
     rep = srv.serve(reqs)
     srv.close()
     print("  report", rep, flush=True)
