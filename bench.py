@@ -270,13 +270,15 @@ def _serve_summary(rep: dict) -> dict:
                 peak_hbm_bytes=rep["peak_hbm_bytes"])
 
 
-def serving_sample(n: int, rates, sizes, max_steps: int, compare_rate: float, omega_max_size: int = 64):
+def serving_sample(n: int, rates, sizes, max_steps: int, compare_rate: float, omega_max_size: int = 64,
+                   seed: int = 3):
     """Serving form: Poisson streams of `n` minimisation requests (pion 60 %, omega 25 %, rho 15 %; L from
     `sizes`; B = 1..8; step limit `max_steps`) served to max|g| < 1e-4 by continuous batching.  The omega
     saddle flow relaxes slowly (up to ~80k steps at 64^3), so omega requests run on lattices of at most
     `omega_max_size`^3 with ``workloads.STEP_BUDGET_FACTOR['omega']`` times the step limit.  At every rate of
     `rates` the FP32 form with each lattice finished in FP64 once max|g| < 3 tol; at `compare_rate` also
-    the plain FP32 and FP64 forms."""
+    the plain FP32 and FP64 forms.  `seed` draws the stream (the default, 3, gives 5 pion, 2 omega and 1 rho
+    request over all four sizes, the nearest 8-request stream to the 60/25/15 mix)."""
     from skyrmion_solver_amd import serving, workloads
 
     runs = []
@@ -285,7 +287,7 @@ def serving_sample(n: int, rates, sizes, max_steps: int, compare_rate: float, om
         if rate == compare_rate:
             forms = [("fp32", _engine.FP32, False), ("fp64", _engine.FP64, False)] + forms
         for name, prec, refine in forms:
-            reqs = workloads.c5_requests(n=n, rate=rate, seed=0, sizes=sizes, max_steps=max_steps, device_init=True,
+            reqs = workloads.c5_requests(n=n, rate=rate, seed=seed, sizes=sizes, max_steps=max_steps, device_init=True,
                                          max_size_by_theory={"omega": omega_max_size})
             srv = serving.Server(precision=prec, hbm_budget_bytes=64 << 30, chunk_steps=32, max_slots=64,
                                  max_blocks=32768, refine_fp64=refine)
@@ -295,7 +297,7 @@ def serving_sample(n: int, rates, sizes, max_steps: int, compare_rate: float, om
             print(f"[bench] serving {name} rate {rate}: converged {rep['converged']}/{rep['requests']}, "
                   f"converged p50 {rep['converged_p50_s']} s, makespan {rep['makespan_s']:.1f} s",
                   file=sys.stderr, flush=True)
-    return dict(requests_per_stream=n, sizes=list(sizes), omega_max_size=omega_max_size, max_steps=max_steps,
+    return dict(requests_per_stream=n, seed=seed, sizes=list(sizes), omega_max_size=omega_max_size, max_steps=max_steps,
                 step_budget_factor=dict(workloads.STEP_BUDGET_FACTOR), tol=1.0e-4, runs=runs)
 
 
@@ -323,6 +325,8 @@ def main():
                     help="step limit of a serving request (omega requests: STEP_BUDGET_FACTOR['omega'] times this)")
     ap.add_argument("--serve-omega-max-size", type=int, default=64,
                     help="largest lattice of an omega serving request")
+    ap.add_argument("--serve-seed", type=int, default=3,
+                    help="seed of the serving streams (3: 5 pion, 2 omega, 1 rho request, the 60/25/15 mix)")
     ap.add_argument("--c3-steps", type=int, default=2000, help="ANF steps of the 160^3 B=8 omega / Coulomb sample "
                                                                "(0: skip)")
     ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
@@ -403,7 +407,7 @@ def main():
     if batch:
         print(f"[bench] batch: {batch}", file=sys.stderr, flush=True)
     serve = serving_sample(args.serve_n, args.serve_rates, args.serve_sizes, args.serve_max_steps,
-                           args.serve_compare_rate, args.serve_omega_max_size) if args.serve_n > 0 else None
+                           args.serve_compare_rate, args.serve_omega_max_size, args.serve_seed) if args.serve_n > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
