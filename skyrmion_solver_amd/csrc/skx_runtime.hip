@@ -162,15 +162,31 @@ LaunchShape stage_tile(const Lattice& L, const LaunchShape& base, int stage, boo
   return wide_tile_stage(L.theory, stage, false) ? kWideTile : base;
 }
 
-// Planes per block along x for one stage kernel: as many x-chunks as keep
-// every block of the launch resident at once (CUs x resident blocks per CU),
-// since a block's run time grows with (chunk + 4) halo-inclusive planes.
+// Planes per block along x for one stage kernel.  A block's run time grows
+// with its (chunk + 4) halo-inclusive planes, and the CUs hold `capacity` =
+// CUs x resident blocks per CU blocks at once, so a launch of nb blocks takes
+// about ceil(nb / capacity) x (chunk + 4) plane times: the chunk count with the
+// least of that (the fewest chunks among equals) is taken.  Filling the
+// resident capacity in one round is that optimum whenever the columns divide
+// it (128^3 on every tile); where they do not, more,
+// shorter chunks in two or more rounds beat a single round that leaves CUs
+// idle (a 192^3 lattice on 32x8 tiles at one block per CU: 144 columns, one
+// 192-plane chunk on 144 CUs against seven 28-plane chunks in 4 rounds).
 int choose_xchunk(const Lattice& L, const LaunchShape& tile, int blocks_per_cu, int n_cu, int forced) {
   if (forced > 0) return forced;
-  const long long cols = (long long)((L.Z + tile.tz - 1) / tile.tz) * ((L.Y + tile.ty - 1) / tile.ty);
+  const long long cols = std::max(1LL, (long long)((L.Z + tile.tz - 1) / tile.tz) * ((L.Y + tile.ty - 1) / tile.ty));
   const long long capacity = (long long)std::max(blocks_per_cu, 1) * std::max(n_cu, 1);
-  const long long nchunks = std::max(1LL, capacity / std::max(cols, 1LL));
-  return (int)std::max(1LL, (L.X + nchunks - 1) / nchunks);
+  long long best_chunk = L.X, best_cost = -1;
+  for (long long n = 1; n <= std::min<long long>(L.X, 64); ++n) {
+    const long long chunk = (L.X + n - 1) / n;
+    const long long nb = cols * ((L.X + chunk - 1) / chunk);
+    const long long cost = ((nb + capacity - 1) / capacity) * (chunk + 4);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best_chunk = chunk;
+    }
+  }
+  return (int)std::max(1LL, best_chunk);
 }
 
 size_t elem_size(int prec) { return prec == SKX_FP64 ? sizeof(double) : sizeof(float); }
