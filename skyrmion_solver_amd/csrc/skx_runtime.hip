@@ -308,8 +308,11 @@ hipEvent_t pooled_event(skx_engine* e) {
     e->ev_pool.pop_back();
     return ev;
   }
+  // timing-only events: no system-scope fence (cache writeback and
+  // invalidate) when they are recorded, so that bracketing a kernel does not
+  // flush the L2 that the next kernel would find warm in an untraced run
   hipEvent_t ev = nullptr;
-  (void)hipEventCreate(&ev);
+  (void)hipEventCreateWithFlags(&ev, hipEventDisableSystemFence);
   return ev;
 }
 
