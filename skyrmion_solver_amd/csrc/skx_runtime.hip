@@ -165,13 +165,13 @@ LaunchShape stage_tile(const Lattice& L, const LaunchShape& base, int stage, boo
 // Planes per block along x for one stage kernel.  A block's run time grows
 // with its (chunk + 4) halo-inclusive planes, and the CUs hold `capacity` =
 // CUs x resident blocks per CU blocks at once, so a launch of nb blocks takes
-// about ceil(nb / capacity) x (chunk + 4) plane times: the chunk count with the
-// least of that (the fewest chunks among equals) is taken.  Filling the
+// about ceil(nb / capacity) x (chunk + 4) plane times: the chunk count with
+// the least of that (the fewest chunks among equals) is taken.  Filling the
 // resident capacity in one round is that optimum whenever the columns divide
-// it (128^3 on every tile); where they do not, more,
-// shorter chunks in two or more rounds beat a single round that leaves CUs
-// idle (a 192^3 lattice on 32x8 tiles at one block per CU: 144 columns, one
-// 192-plane chunk on 144 CUs against seven 28-plane chunks in 4 rounds).
+// it (128^3 on every tile).  Where they do not, more and shorter chunks in
+// two or more rounds beat one round that leaves CUs idle: a 192^3 lattice on
+// 32x8 tiles at one block per CU has 144 columns, and one 192-plane chunk per
+// column runs on 144 CUs, against seven 28-plane chunks in 4 rounds.
 int choose_xchunk(const Lattice& L, const LaunchShape& tile, int blocks_per_cu, int n_cu, int forced) {
   if (forced > 0) return forced;
   const long long cols = std::max(1LL, (long long)((L.Z + tile.tz - 1) / tile.tz) * ((L.Y + tile.ty - 1) / tile.ty));
