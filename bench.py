@@ -221,6 +221,22 @@ def batch_sample(n: int, steps: int):
                 blocks=blocks)
 
 
+THEORY_IDS = {"pion": _engine.THEORY_PION, "omega": _engine.THEORY_OMEGA, "coulomb": _engine.THEORY_COULOMB}
+
+
+def c3_workload(theory: str, L: int = 160):
+    """The omega / Coulomb B=8 product workload of the 160^3 sample on an L^3 lattice, as a host field
+    (omega starting at -c_omega CS); returns (rp, p_i, p_f, F)."""
+    from skyrmion_solver_amd import workloads
+
+    _, rp, p_i, p_f, F = workloads.c3_b8(theory, L=L)
+    if theory == "omega":
+        with _engine.Engine(_engine.THEORY_OMEGA, _engine.FP64, p_i, p_f) as eng:
+            eng.upload(_engine.FIELD, F)
+            F = workloads.set_omega_from_chern_simons(eng, rp)
+    return rp, p_i, p_f, F
+
+
 def c3_sample(steps: int = 2000, theories=("omega", "coulomb")):
     """160^3 B=8 omega and Coulomb (product of two B=4 rational maps at -+1.5 along x, dx = 0.15; fields
     built on the device): `steps` ANF steps in the FP64 and FP32 forms, each timed as one device-resident
@@ -332,10 +348,13 @@ def main():
     ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
                     help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
     ap.add_argument("--size", type=int, default=128,
-                    help="profiling aid with --only fp32/fp64: lattice edge of the pion B=4 workload")
+                    help="profiling aid with --only fp32/fp64: lattice edge of the workload")
+    ap.add_argument("--theory", choices=["pion", "omega", "coulomb"], default="pion",
+                    help="profiling aid with --only fp32/fp64: the pion B=4 workload, or the omega / Coulomb "
+                         "B=8 product workload of the 160^3 sample")
     args = ap.parse_args()
-    if args.size != 128 and args.only not in ("fp32", "fp64"):
-        ap.error("--size applies to --only fp32 / fp64 runs; the metric is quoted at 128^3")
+    if (args.size != 128 or args.theory != "pion") and args.only not in ("fp32", "fp64"):
+        ap.error("--size and --theory apply to --only fp32 / fp64 runs; the metric is quoted at 128^3 pion")
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is design-only)"}))
         sys.exit(2)
@@ -344,12 +363,16 @@ def main():
         gbs, ms = _engine.stream_copy_bandwidth(1 << 30, 20)
         print(f"[bench] stream copy of 1 GiB: {gbs:.1f} GB/s, {ms:.4f} ms per copy", file=sys.stderr, flush=True)
         return
-    rp, p_i, p_f, F = workload(L=args.size)
+    if args.only in ("fp32", "fp64") and args.theory != "pion":
+        rp, p_i, p_f, F = c3_workload(args.theory, args.size)
+    else:
+        rp, p_i, p_f, F = workload(L=args.size)
     N = int(p_i[6])
     nf = 4
     if args.only in ("fp32", "fp64"):
         prec = _engine.FP32 if args.only == "fp32" else _engine.FP64
-        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk, trace=not args.no_trace)
+        r = time_form(prec, p_i, p_f, F, args.steps, args.warmup, args.chunk, trace=not args.no_trace,
+                      theory=THEORY_IDS[args.theory])
         print(f"[bench] {args.only}: {N * r['steps'] / r['seconds']:.4g} site-updates/s, "
               f"{1000.0 * r['seconds'] / r['steps']:.4f} ms/step", file=sys.stderr, flush=True)
         for k, v in r.get("kernels", {}).items():
