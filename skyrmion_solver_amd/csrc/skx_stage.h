@@ -129,9 +129,14 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   constexpr bool AOS = (NST == 4);
   constexpr int ZC = TR * TZ;  // cells of one z-difference plane (tile rows with halo x TZ)
   __shared__ __align__(16) T fring[FSLOTS * NST * TE];  // (rawq: declared below, stage 4 only)
-  __shared__ __align__(16) T zring[ANY_MIXED ? ZSLOTS * NST * ZC : 1];
+  // the components with mixed derivatives are the leading NMIX stencil
+  // components (Traits::MIXED = 2^NMIX - 1); only they have a z-difference ring
+  static_assert((Tr::MIXED & (Tr::MIXED + 1)) == 0, "mixed components must be the leading ones");
+  constexpr int NMIX = (Tr::MIXED & 8u) ? 4 : (Tr::MIXED & 4u) ? 3 : (Tr::MIXED & 2u) ? 2 : (Tr::MIXED & 1u) ? 1 : 0;
+  static_assert(NMIX <= NST && (AOS ? NMIX == NST || NMIX == 0 : true), "AOS z-ring holds every stencil component");
+  __shared__ __align__(16) T zring[ANY_MIXED ? ZSLOTS * NMIX * ZC : 1];
   auto fi = [&](int slot, int a, int e) { return AOS ? (slot * TE + e) * NST + a : (slot * NST + a) * TE + e; };
-  auto zi = [&](int slot, int a, int e) { return AOS ? (slot * ZC + e) * NST + a : (slot * NST + a) * ZC + e; };
+  auto zi = [&](int slot, int a, int e) { return AOS ? (slot * ZC + e) * NMIX + a : (slot * NMIX + a) * ZC + e; };
   __shared__ double red[2 * (NT / 64)];
 
   if ((A.flags & CHECK_DONE) && A.st->done) return;  // uniform per launch
@@ -219,12 +224,17 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   // plane p arrives: it is read, then overwritten with plane p.  Each thread
   // touches only its own entries, so the ring needs no barrier.  Used where it
   // fits in the 160 KB of LDS beside the plane rings (all FP32 shapes; FP64
-  // pion 32x8 and 16x16, where one block per CU is resident anyway).
-  constexpr size_t LDS_RINGS = sizeof(T) * (size_t(FSLOTS) * NST * TE + (ANY_MIXED ? size_t(ZSLOTS) * NST * TR * TZ : 0)) +
+  // pion 32x8 and 16x16, where one block per CU is resident anyway), for
+  // every stencil component or, where that does not fit, for the first four
+  // (FP64 Coulomb: its fifth component, V, is then read at the centre).
+  constexpr size_t LDS_RINGS = sizeof(T) * (size_t(FSLOTS) * NST * TE + (ANY_MIXED ? size_t(ZSLOTS) * NMIX * TR * TZ : 0)) +
                                2 * (NT / 64) * sizeof(double);
-  constexpr size_t LDS_RAWQ = sizeof(T) * 2 * 3 * size_t(NST) * NT;
-  constexpr bool RAWQ = (STAGE == 4) && (sizeof(T) == 4 || LDS_RINGS + LDS_RAWQ <= 160 * 1024);
-  __shared__ __align__(16) T rawq[RAWQ ? 2 * NT * 3 * NST : 1];  // [slot][thread][F, V, l2 x NST]
+  constexpr size_t LDS_MAX = 160 * 1024;
+  constexpr bool RAWQ_ALL = sizeof(T) == 4 || LDS_RINGS + sizeof(T) * 2 * 3 * size_t(NST) * NT <= LDS_MAX;
+  constexpr bool RAWQ_FOUR = NST > 4 && LDS_RINGS + sizeof(T) * 2 * 3 * size_t(4) * NT <= LDS_MAX;
+  constexpr bool RAWQ = (STAGE == 4) && (RAWQ_ALL || RAWQ_FOUR);
+  constexpr int NRQ = RAWQ_ALL ? NST : 4;  // components in the ring
+  __shared__ __align__(16) T rawq[RAWQ ? 2 * NT * 3 * NRQ : 1];  // [slot][thread][F, V, l2 x NRQ]
 
   double acc0 = (STAGE == 4) ? -1.0e300 : 0.0;
   double acc1 = 0.0;
@@ -252,20 +262,20 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
     }
     T cF[NF], cV[NF], cl1[NF], cl2[NF], cl3[NF];
     if constexpr (RAWQ) {
-      T* rq = rawq + ((p & 1) * NT + tid) * (3 * NST);
+      T* rq = rawq + ((p & 1) * NT + tid) * (3 * NRQ);
       if constexpr (HC) {
 #pragma unroll
-        for (int a = 0; a < NST; ++a) {
+        for (int a = 0; a < NRQ; ++a) {
           cF[a] = rq[a];
-          cV[a] = rq[NST + a];
-          cl2[a] = rq[2 * NST + a];
+          cV[a] = rq[NRQ + a];
+          cl2[a] = rq[2 * NRQ + a];
         }
       }
 #pragma unroll
-      for (int a = 0; a < NST; ++a) {
+      for (int a = 0; a < NRQ; ++a) {
         rq[a] = buf[0][a][0];
-        rq[NST + a] = buf[1][a][0];
-        rq[2 * NST + a] = buf[NA > 2 ? 2 : 0][a][0];
+        rq[NRQ + a] = buf[1][a][0];
+        rq[2 * NRQ + a] = buf[NA > 2 ? 2 : 0][a][0];
       }
     }
 
@@ -281,7 +291,7 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
 #pragma unroll
       for (int a = 0; a < NF; ++a) {
         const size_t o = a * N + pb;
-        if (!RAWQ || a >= NST) {
+        if (!RAWQ || a >= NRQ) {
           cF[a] = bload<T>(plane_rsrc(A.F + o, plane_bytes), coff);
           cV[a] = bload<T>(plane_rsrc(A.V + o, plane_bytes), coff);
           cl2[a] = bload<T>(plane_rsrc(A.la + o, plane_bytes), coff);
