@@ -32,15 +32,18 @@ HIPCC_FLAGS = [
     "-ffp-contract=on",
 ]
 
-# Per-source additions.  The FP32 pion stage kernels take the ILP-first
-# instruction scheduler: it spends their register headroom at 2 waves per SIMD
-# on more independent work (stage 3 -5%, stage 4 -2%, whole step +0.9% at
-# 128^3), with no spill or occupancy loss in any of their instantiations.
-# Elsewhere it costs a wave (lightly bound stage 2, the rho curl pass) or
-# time (FP64), so it is not applied there.  Scheduling only orders
-# instructions: results are bit-identical either way.
+# Per-source additions.  The FP32 pion and Coulomb stage kernels take the
+# ILP-first instruction scheduler: it spends their register headroom on more
+# independent work with no spill or occupancy loss in any of their
+# instantiations (pion: stage 3 -5%, stage 4 -2%, whole step +0.9% at 128^3;
+# Coulomb: step -1.4% at 128^3, -2.0% at 160^3).  Elsewhere it costs a wave
+# (lightly bound stage 2, the rho curl pass, omega stage 3) or time (FP64), so
+# it is not applied there.  Scheduling only orders instructions: results are
+# bit-identical either way.
+_MAX_ILP = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
 SOURCE_FLAGS = {
-    "skx_stage_pion_f32.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
+    "skx_stage_pion_f32.hip": _MAX_ILP,
+    "skx_stage_coulomb_f32.hip": _MAX_ILP,
 }
 
 
