@@ -19,9 +19,10 @@
 // (reference x) over a chunk of planes.  Each incoming plane of the stage
 // field (with a 2-site y/z halo) is staged in a 4-slot LDS ring, so the
 // centre plane (two behind the lead) is still resident for its y/z second
-// differences; one barrier per plane.  Per-site values along x (the field
-// and its raw y / z first differences) live in 5-deep register queues, and
-// raw z-differences of every tile row go to a 3-slot LDS ring for the yz
+// differences; one barrier per plane.  Per-site values along x (the field,
+// and the raw y / z first differences of the components with mixed
+// derivatives) live in 5-deep register queues, and the raw z-differences of
+// every tile row of those components go to a 3-slot LDS ring for the yz
 // mixed derivative.  Mixed derivatives are first differences of raw first
 // differences, which is the reference's 25-tap tensor stencil
 // (core/derivatives.py:121-173) regrouped.
