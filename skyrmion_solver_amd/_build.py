@@ -32,19 +32,22 @@ HIPCC_FLAGS = [
     "-ffp-contract=on",
 ]
 
-# Per-source additions.  The FP32 pion, Coulomb and Berger stage kernels take
-# the ILP-first instruction scheduler: it spends their register headroom on
-# more independent work with no spill or occupancy loss in any of their
-# instantiations (pion: stage 3 -5%, stage 4 -2%, whole step +0.9% at 128^3;
-# Coulomb: step -1.4% at 128^3, -2.0% at 160^3; Berger: -1.4 to -2.0% at
-# 128^3 and 160^3).  Elsewhere it costs a wave (lightly bound stage 2, the rho
-# curl pass, omega stage 3) or time (FP64), so it is not applied there.
-# Scheduling only orders instructions: results are bit-identical either way.
+# Per-source additions.  The FP32 pion, Coulomb, Berger and lightly bound
+# stage kernels take the ILP-first instruction scheduler: it spends their
+# register headroom on more independent work with no spill, and no occupancy
+# loss in the kernels their default launch shapes use (pion: stage 3 -5%,
+# stage 4 -2%, whole step +0.9% at 128^3; Coulomb: step -1.4% at 128^3, -2.0%
+# at 160^3; Berger: -1.4 to -2.0%; lightly bound: -0.1 to -1.5%; its energy
+# pass on a uniform 32x8 tile drops from 3 to 2 waves per SIMD).  For omega
+# (stage 3) and the rho curl pass it costs a wave, and FP64 pion measured 1.7%
+# slower, so it is not applied there.  Scheduling only orders instructions:
+# results are bit-identical either way.
 _MAX_ILP = ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]
 SOURCE_FLAGS = {
     "skx_stage_pion_f32.hip": _MAX_ILP,
     "skx_stage_coulomb_f32.hip": _MAX_ILP,
     "skx_stage_berger_f32.hip": _MAX_ILP,
+    "skx_stage_lightly_f32.hip": _MAX_ILP,
 }
 
 
