@@ -464,6 +464,7 @@ def main():
         "serving": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "device": dict(zip(("arch", "compute_units"), _engine.device_info(0))),
     }
     print(json.dumps(line), flush=True)
 
