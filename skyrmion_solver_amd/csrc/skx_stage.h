@@ -123,21 +123,33 @@ __device__ __forceinline__ void stage_body(const StageArgs<T>& A, int blk, int n
   static_assert(NT % 64 == 0, "block must be whole wavefronts");
   static_assert(TY >= 4, "halo rows of the z-difference ring are filled by the first four rows of threads");
 
-  // LDS layout of the rings.  With four stencil components (AOS) the
-  // components of one cell are contiguous, so a cell moves in one 16-byte
-  // (FP32) access and every neighbour is the thread's own cell address plus
-  // a constant; otherwise each component is a plane of its own.
-  constexpr bool AOS = (NST == 4);
+  // LDS layout of the rings.  In the AOS layout the four leading components
+  // of one cell are contiguous, so they move in one 16-byte (FP32) access and
+  // every neighbour is the thread's own cell address plus a constant; a fifth
+  // stencil component is a plane of its own after the cells.  Otherwise each
+  // component is a plane of its own.  AOS for every four-component theory;
+  // for the five-component ones only in the FP32 stages where it measured
+  // faster (wider accesses against more registers held: omega stages 1 and 4
+  // 60.4 -> 58.4 and 151.3 -> 148.6 us at 160^3, Coulomb stage 2 43.7 -> 41.5
+  // us at 128^3; on Coulomb's stages 3 and 4 it spilled).
+  constexpr bool AOS5 = (NST == 5) && sizeof(T) == 4 &&
+                        ((TH == COULOMB && STAGE == 2) || (TH == OMEGA && (STAGE == 1 || STAGE == 4)));
+  constexpr bool AOS = (NST == 4) || AOS5;
   constexpr int ZC = TR * TZ;  // cells of one z-difference plane (tile rows with halo x TZ)
   __shared__ __align__(16) T fring[FSLOTS * NST * TE];  // (rawq: declared below, stage 4 only)
   // the components with mixed derivatives are the leading NMIX stencil
   // components (Traits::MIXED = 2^NMIX - 1); only they have a z-difference ring
   static_assert((Tr::MIXED & (Tr::MIXED + 1)) == 0, "mixed components must be the leading ones");
   constexpr int NMIX = (Tr::MIXED & 8u) ? 4 : (Tr::MIXED & 4u) ? 3 : (Tr::MIXED & 2u) ? 2 : (Tr::MIXED & 1u) ? 1 : 0;
-  static_assert(NMIX <= NST && (AOS ? NMIX == NST || NMIX == 0 : true), "AOS z-ring holds every stencil component");
+  static_assert(NMIX <= 4, "the z-difference ring holds at most the four leading components");
   __shared__ __align__(16) T zring[ANY_MIXED ? ZSLOTS * NMIX * ZC : 1];
-  auto fi = [&](int slot, int a, int e) { return AOS ? (slot * TE + e) * NST + a : (slot * NST + a) * TE + e; };
-  auto zi = [&](int slot, int a, int e) { return AOS ? (slot * ZC + e) * NMIX + a : (slot * NMIX + a) * ZC + e; };
+  auto fi = [&](int slot, int a, int e) {
+    if (!AOS) return (slot * NST + a) * TE + e;
+    return a < 4 ? (slot * TE + e) * 4 + a : FSLOTS * 4 * TE + (slot * (NST - 4) + (a - 4)) * TE + e;
+  };
+  auto zi = [&](int slot, int a, int e) {
+    return (AOS && NMIX == 4) ? (slot * ZC + e) * 4 + a : (slot * NMIX + a) * ZC + e;
+  };
   __shared__ double red[2 * (NT / 64)];
 
   if ((A.flags & CHECK_DONE) && A.st->done) return;  // uniform per launch
