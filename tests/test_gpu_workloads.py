@@ -1,7 +1,9 @@
-"""GPU: the benchmark workloads (32^3 B=1 hedgehog trajectory against the
+"""GPU: the benchmark workloads (32^3 B=1 hedgehog trajectories against the
 oracle; 160^3 B=8 omega / Coulomb FP32 form against the FP64 form)."""
 
 from __future__ import annotations
+
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -37,6 +39,39 @@ def test_c1_hedgehog_trajectory_matches_oracle():
     assert abs(stats.energy - ref[-1][0]) <= 1e-12 * abs(ref[-1][0])
     assert rel_l2(Fg, st.F) < 1e-11
     assert rel_l2(Vg, st.V) < 1e-9 or np.max(np.abs(st.V)) == 0.0
+
+
+C1_FIXTURE = Path(__file__).resolve().parent / "golden" / "c1_trajectories.npz"
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_c1_500_steps_match_oracle_trajectory(seed):
+    """32^3 B=1 hedgehog, FP64, seeds 0..9, 500 ANF steps in one device-resident run, against the oracle's
+    trajectories (tests/golden/c1_trajectories.npz, made by make_c1_trajectories.py there): energy, baryon
+    number, signed err and max|g| of every step, the steps that arrested, the final field at 512 sites
+    and its per-component sums and sums of squares."""
+    print(f"[case] c1 hedgehog 32^3 fp64 500 steps seed {seed}", flush=True)
+    d = np.load(C1_FIXTURE)
+    th, rp, p_i, p_f, F = workloads.c1_hedgehog(seed=seed)
+    with _engine.Engine(_engine.THEORY_PION, _engine.FP64, p_i, p_f) as eng:
+        eng.upload(_engine.FIELD, F)
+        eng.history_attach(512)
+        stats = eng.anf_run(500, -1.0, 100)
+        hist = np.array(eng.history_read(512))
+        Fg = eng.download(_engine.FIELD).reshape(4, -1)
+    assert stats.steps == 500 and hist.shape == (500, 4)
+    E_ref, B_ref = d["energy"][seed], d["baryon"][seed]
+    E = np.append(hist[:, 0], stats.energy)
+    B = np.append(hist[:, 3], stats.baryon)
+    assert np.max(np.abs(E - E_ref)) <= 1e-12 * np.max(np.abs(E_ref))
+    assert np.max(np.abs(B - B_ref)) <= 1e-12
+    assert np.max(np.abs(hist[:, 1] - d["err"][seed]) / np.abs(d["err"][seed])) <= 1e-10
+    assert np.max(np.abs(hist[:, 2] - d["max_force"][seed]) / d["max_force"][seed]) <= 1e-10
+    assert np.array_equal(np.diff(E) > 0.0, d["arrested"][seed])
+    sample = Fg[:, d["sites"][seed]]
+    assert rel_l2(sample, d["final_sample"][seed]) < 1e-11
+    assert np.max(np.abs(Fg.sum(axis=1) - d["final_sum"][seed])) <= 1e-9
+    assert np.max(np.abs((Fg * Fg).sum(axis=1) / d["final_sumsq"][seed] - 1.0)) <= 1e-12
 
 
 def test_c1_full_run_keeps_b1():

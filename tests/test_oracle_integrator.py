@@ -11,6 +11,8 @@
 from __future__ import annotations
 
 import math
+import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -148,3 +150,38 @@ def test_slab_parallel_step_equals_the_oracle_bitwise(theory):
     finally:
         pool.close()
     assert np.array_equal(a.F, b.F) and np.array_equal(a.V, b.V)
+
+
+def _c1_fixture_module():
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    import make_c1_trajectories
+
+    return make_c1_trajectories
+
+
+def test_c1_trajectory_fixture_is_the_oracles():
+    """tests/golden/c1_trajectories.npz (the 500-step C1 trajectories the GPU test compares against) holds
+    the inputs the workload builder makes today and the oracle's first steps from them; its records are
+    consistent: an arrest exactly where the energy rose, B near 1, |err| <= max|g|."""
+    from skyrmion_solver_amd import workloads
+
+    mk = _c1_fixture_module()
+    d = np.load(mk.OUT)
+    assert tuple(d["seeds"]) == mk.SEEDS and d["energy"].shape == (len(mk.SEEDS), mk.STEPS + 1)
+    for s in mk.SEEDS:
+        _th, _rp, p_i, p_f, F = workloads.c1_hedgehog(seed=s)
+        assert mk.input_hash(F) == d["input_sha"][s], s
+        rose = np.diff(d["energy"][s]) > 0.0
+        assert np.array_equal(rose, d["arrested"][s]), s
+        assert np.all(np.abs(d["err"][s]) <= d["max_force"][s])
+        assert abs(d["baryon"][s, -1] - 1.0) < 0.01
+        assert np.array_equal(d["sites"][s], mk.sample_sites(s, F.size // 4))
+    _th, _rp, p_i, p_f, F = workloads.c1_hedgehog(seed=0)
+    st = OI.OracleState("pion", p_i, p_f, F)
+    E = OI.energy(st)
+    assert abs(E - d["energy"][0, 0]) <= 1e-13 * abs(E)
+    for k in range(3):
+        E, err = OI.anf_step(st, E)
+        assert abs(E - d["energy"][0, k + 1]) <= 1e-13 * abs(E)
+        assert abs(err - d["err"][0, k]) <= 1e-12 * abs(err)
+        assert abs(OI.max_abs_force(st) - d["max_force"][0, k]) <= 1e-12 * d["max_force"][0, k]
