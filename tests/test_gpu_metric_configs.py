@@ -3,7 +3,7 @@
 * 128^3 B=4 pion (the headline workload): two FP64 ANF steps against the
   FP64 oracle (core/integrator.py:284-410 with
   theories/nuclear_skyrme/kernels.py:1034-1150), and the FP32 form relaxed
-  to max|g| < 1e-4 against the FP64 form relaxed the same way.
+  to max|g| < 1e-4 against the FP64 form relaxed the same way, seeds 0..19.
 * The 48-lattice batch ({48, 64, 96, 128}^3, B = 1..6, m in [0, 1.5]) with
   the batch's own launch shape (a tile per stage, ~32-plane x-chunks): every
   lattice bit-identical to its solo run after several chunks.
@@ -67,27 +67,30 @@ def test_c2_fp64_steps_match_oracle():
 
 
 def test_c2_fp32_form_converges_to_the_fp64_solution():
-    """128^3 B=4, both forms relaxed to max|g| < 1e-4.  The FP32 form against the FP64 form (which matches
-    the oracle step for step above): |dE|/E <= 1e-6, |dB| <= 1e-4, field rel-L2 <= 1e-4."""
-    print("[case] 128^3 B=4 pion fp32 vs fp64 to convergence", flush=True)
-    th, rp, p_i, p_f, F = workloads.c2_b4(seed=0)
-    out = {}
-    for prec in (_engine.FP64, _engine.FP32):
-        with _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f) as eng:
-            eng.upload(_engine.FIELD, F)
-            st = eng.anf_run(20000, 1.0e-4, 64)
-            out[prec] = (st, eng.download(_engine.FIELD))
-    (s64, F64), (s32, F32) = out[_engine.FP64], out[_engine.FP32]
-    dE = abs(s32.energy - s64.energy) / abs(s64.energy)
-    dB = abs(s32.baryon - s64.baryon)
-    L2 = rel_l2(F32, F64)
-    print(f"  steps {s64.steps}/{s32.steps} E {s64.energy:.12g}/{s32.energy:.12g} dE/E {dE:.2e} dB {dB:.2e} "
-          f"L2 {L2:.2e}", flush=True)
-    assert s64.converged == 1 and s32.converged == 1
-    assert abs(s64.baryon - 4.0) < 0.05
-    assert dE <= 1e-6
-    assert dB <= 1e-4
-    assert L2 <= 1e-4
+    """128^3 B=4, seeds 0..19 (the noise of each start), both forms relaxed to max|g| < 1e-4.  The FP32
+    form against the FP64 form (which matches the oracle step for step above), every seed: |dE|/E <= 1e-6,
+    |dB| <= 1e-4, field rel-L2 <= 1e-4."""
+    worst = dict(dE=0.0, dB=0.0, L2=0.0)
+    for seed in range(20):
+        print(f"[case] 128^3 B=4 pion fp32 vs fp64 to convergence, seed {seed}", flush=True)
+        th, rp, p_i, p_f, F = workloads.c2_b4(seed=seed)
+        out = {}
+        for prec in (_engine.FP64, _engine.FP32):
+            with _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f) as eng:
+                eng.upload(_engine.FIELD, F)
+                st = eng.anf_run(20000, 1.0e-4, 64)
+                out[prec] = (st, eng.download(_engine.FIELD))
+        (s64, F64), (s32, F32) = out[_engine.FP64], out[_engine.FP32]
+        dE = abs(s32.energy - s64.energy) / abs(s64.energy)
+        dB = abs(s32.baryon - s64.baryon)
+        L2 = rel_l2(F32, F64)
+        print(f"  steps {s64.steps}/{s32.steps} E {s64.energy:.12g}/{s32.energy:.12g} dE/E {dE:.2e} dB {dB:.2e} "
+              f"L2 {L2:.2e}", flush=True)
+        assert s64.converged == 1 and s32.converged == 1, seed
+        assert abs(s64.baryon - 4.0) < 0.05, seed
+        assert dE <= 1e-6 and dB <= 1e-4 and L2 <= 1e-4, seed
+        worst = dict(dE=max(worst["dE"], dE), dB=max(worst["dB"], dB), L2=max(worst["L2"], L2))
+    print(f"  worst over 20 seeds: dE/E {worst['dE']:.2e} dB {worst['dB']:.2e} L2 {worst['L2']:.2e}", flush=True)
 
 
 def _oracle_threads() -> int:
