@@ -88,7 +88,8 @@ def test_c1_full_run_keeps_b1():
 
 @pytest.mark.parametrize("theory", ["omega", "coulomb"])
 def test_c3_b8_fp32_form_against_fp64(theory):
-    """160^3 B=8: 200 ANF steps; FP32 form vs FP64: |dE|/E <= 1e-5, |dB| <= 1e-4, field rel-L2 <= 1e-4."""
+    """160^3 B=8: 2000 ANF steps; FP32 form vs FP64: |dE| <= 1e-6 of the run's energy scale, |dB| <= 1e-4,
+    field rel-L2 <= 1e-4."""
     print(f"[case] c3 {theory} 160^3", flush=True)
     th, rp, p_i, p_f, F = workloads.c3_b8(theory)
     tid = _engine.THEORY_OMEGA if theory == "omega" else _engine.THEORY_COULOMB
@@ -98,15 +99,17 @@ def test_c3_b8_fp32_form_against_fp64(theory):
             eng.upload(_engine.FIELD, F)
             if theory == "omega":
                 workloads.set_omega_from_chern_simons(eng, rp)
-            B0 = eng.baryon()
-            st = eng.anf_run(200, -1.0, 100)
-            out[prec] = (st, eng.download(_engine.FIELD), B0)
-    s64, F64, B0 = out[_engine.FP64]
-    s32, F32, _ = out[_engine.FP32]
-    print(f"  B0={B0:.6f} E64={s64.energy:.10g} E32={s32.energy:.10g} B64={s64.baryon:.8f} B32={s32.baryon:.8f} "
-          f"L2={rel_l2(F32, F64):.3e}", flush=True)
+            B0, E0 = eng.baryon(), eng.energy()
+            st = eng.anf_run(2000, -1.0, 100)
+            out[prec] = (st, eng.download(_engine.FIELD), B0, E0)
+    s64, F64, B0, E0 = out[_engine.FP64]
+    s32, F32, _, _ = out[_engine.FP32]
+    print(f"  B0={B0:.6f} E0={E0:.10g} E64={s64.energy:.10g} E32={s32.energy:.10g} B64={s64.baryon:.8f} "
+          f"B32={s32.baryon:.8f} L2={rel_l2(F32, F64):.3e}", flush=True)
+    assert s64.steps == s32.steps == 2000
     assert abs(B0 - 8.0) < 0.2
-    assert abs(s32.energy - s64.energy) <= 1e-5 * abs(s64.energy)
+    # the omega soliton at c_omega = 0.5 unwinds (E -> ~1e-7, B -> 0): the energy is compared on the run's scale
+    assert abs(s32.energy - s64.energy) <= 1e-6 * max(abs(E0), abs(s64.energy))
     assert abs(s32.baryon - s64.baryon) <= 1e-4
     assert rel_l2(F32, F64) <= 1e-4
 
