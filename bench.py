@@ -182,20 +182,24 @@ def cpu_baseline(p_i, p_f, F, steps: int = 3, threads: int | None = None, thread
                 host=host)
 
 
-def batch_sample(n: int, steps: int):
+def batch_sample(n: int, steps: int, shape=None, solo: bool = True):
     """Batched form: `n` independent pion lattices (L uniform in {48, 64, 96, 128}^3, B = 1..6 rational maps
     with random orientations, m ~ U[0, 1.5]) advanced together, FP32.  The batch's tile and x-chunk rule
-    come from the autotuner (timed on one lattice per size class); the comparison is the same lattices run
-    one after another as solo engines with the engine's default launch shape."""
+    come from the autotuner (timed on one lattice per size class), or are `shape` = (tile_z, x-chunk
+    target) when given; the comparison (`solo`) is the same lattices run one after another as solo engines
+    with the engine's default launch shape."""
     from skyrmion_solver_amd import autotune, workloads
 
     inst = workloads.c4_instances(n=n, seed=0, device_init=True)
     per_class = {}
     for d in inst:
         per_class.setdefault(d["L"], d)
-    tuned = autotune.tune_batch(_engine.THEORY_PION, _engine.FP32,
-                                [(d["p_i"], d["p_f"]) for d in per_class.values()], steps=8, use_cache=False)
-    tz, tgt = tuned["tile_z"], tuned["target"]
+    if shape is None:
+        tuned = autotune.tune_batch(_engine.THEORY_PION, _engine.FP32,
+                                    [(d["p_i"], d["p_f"]) for d in per_class.values()], steps=8, use_cache=False)
+        tz, tgt = tuned["tile_z"], tuned["target"]
+    else:
+        tz, tgt = shape
     dims = [(d["L"],) * 3 for d in inst]
     blocks = autotune.batch_blocks(dims, tz, tgt)
     sites = sum(d["L"] ** 3 for d in inst)
@@ -206,6 +210,9 @@ def batch_sample(n: int, steps: int):
         t0 = time.perf_counter()
         b.run(steps)
         dt = time.perf_counter() - t0
+    if not solo:
+        return dict(instances=len(inst), sites=sites, steps=steps, site_updates_per_s=sites * steps / dt,
+                    tile_z=tz, x_chunk_target=tgt, blocks=blocks)
     # solo baseline: one engine per size class, scaled by the class count
     solo_s = 0.0
     for L, d in per_class.items():
@@ -345,7 +352,9 @@ def main():
                     help="seed of the serving streams (3: 5 pion, 2 omega, 1 rho request, the 60/25/15 mix)")
     ap.add_argument("--c3-steps", type=int, default=2000, help="ANF steps of the 160^3 B=8 omega / Coulomb sample "
                                                                "(0: skip)")
-    ap.add_argument("--only", choices=["fp32", "fp64", "copy"], default=None,
+    ap.add_argument("--batch-shape", type=int, nargs=2, default=None, metavar=("TILE_Z", "TARGET"),
+                    help="batch tile and x-chunk target instead of the autotuner's (profiling aid)")
+    ap.add_argument("--only", choices=["fp32", "fp64", "copy", "batch"], default=None,
                     help="profiling aid: run only one form (or only the copy calibration) and print no JSON")
     ap.add_argument("--size", type=int, default=128,
                     help="profiling aid with --only fp32/fp64: lattice edge of the workload")
@@ -362,6 +371,11 @@ def main():
     if args.only == "copy":
         gbs, ms = _engine.stream_copy_bandwidth(1 << 30, 20)
         print(f"[bench] stream copy of 1 GiB: {gbs:.1f} GB/s, {ms:.4f} ms per copy", file=sys.stderr, flush=True)
+        return
+    if args.only == "batch":
+        b = batch_sample(args.batch_n, args.batch_steps, shape=args.batch_shape, solo=False)
+        print(f"[bench] batch: {b['site_updates_per_s']:.4g} site-updates/s over {b['instances']} lattices, "
+              f"{1000.0 * b['sites'] / b['site_updates_per_s']:.4f} ms/step, {b}", file=sys.stderr, flush=True)
         return
     if args.only in ("fp32", "fp64") and args.theory != "pion":
         rp, p_i, p_f, F = c3_workload(args.theory, args.size)
