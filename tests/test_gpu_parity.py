@@ -44,6 +44,40 @@ def test_gradient_probe_fp64(theory, shape):
     assert np.max(np.abs(eg - EG_ref.ravel())) <= 1e-10 * np.max(np.abs(EG_ref))
 
 
+# thinner than the stencil along some axes, down to one site (every site in the boundary band)
+DEGENERATE = [(1, 1, 1), (2, 3, 4), (4, 4, 4), (5, 1, 6), (6, 5, 1), (1, 33, 9), (7, 2, 40)]
+
+
+@pytest.mark.parametrize("prec", [_engine.FP64, _engine.FP32])
+@pytest.mark.parametrize("theory", ["pion", "omega", "rho"])
+@pytest.mark.parametrize("shape", DEGENERATE)
+def test_degenerate_lattices(shape, theory, prec):
+    """Lattices with axes shorter than the five-point stencil, down to 1x1x1: the gradient, and three ANF
+    steps of the device loop, against the FP64 oracle (FP64: 1e-12; FP32 storage: 1e-5 of the run's scale)."""
+    mod, rp, p_i, p_f = make_params(theory, *shape)
+    nf = int(p_i[5])
+    T = smooth_field(rp, nf, seed=5, normalise=True)
+    st = OI.OracleState(theory, p_i, p_f, T)
+    EG_ref, _ = OI.gradient(st, st.F)
+    E = E0 = OI.energy(st)
+    ref = []
+    for _ in range(3):
+        E, err = OI.anf_step(st, E)
+        ref.append((E, err))
+    with _engine_for(theory, rp, p_i, p_f, prec=prec) as eng:
+        eg, _ = eng.eval_gradient(T)
+        eng.upload(_engine.FIELD, T)
+        run = eng.anf_run(3, -1.0, 3)
+        Fg = eng.download(_engine.FIELD)
+    tol = 1e-12 if prec == _engine.FP64 else 1e-5
+    assert np.max(np.abs(eg - EG_ref.ravel())) <= tol * np.max(np.abs(EG_ref))
+    assert run.steps == 3
+    escale = max(abs(E0), max(abs(e) for e, _ in ref))
+    assert abs(run.energy - ref[-1][0]) <= tol * escale
+    assert abs(run.err_signed - ref[-1][1]) <= (1e-10 if prec == _engine.FP64 else 1e-5) * abs(ref[-1][1])
+    assert rel_l2(Fg, st.F) < (1e-12 if prec == _engine.FP64 else 1e-5)
+
+
 @pytest.mark.parametrize("theory", ["pion", "rho"])
 @pytest.mark.parametrize("tile_z", [16, 32, 64])
 def test_gradient_probe_tiles(tile_z, theory):

@@ -92,6 +92,18 @@ def test_invalid_descriptor_is_rejected_without_a_device():
     assert e.value.code == -1
     n = _engine.footprint_bytes(_engine.THEORY_PION, _engine.FP32, p_i, p_f)
     assert n >= 7 * 4 * 512 * 4
+    # an empty lattice, a size that disagrees with its dimensions, a non-positive spacing
+    for k, v in ((0, 0), (6, 511), (7, 4 * 511)):
+        bad = p_i.copy()
+        bad[k] = v
+        with pytest.raises(_engine.SkxError) as e:
+            _engine.footprint_bytes(_engine.THEORY_PION, _engine.FP64, bad, p_f)
+        assert e.value.code == -1, k
+    bad_f = p_f.copy()
+    bad_f[4] = 0.0
+    with pytest.raises(_engine.SkxError) as e:
+        _engine.footprint_bytes(_engine.THEORY_PION, _engine.FP64, p_i, bad_f)
+    assert e.value.code == -1
 
 
 def test_host_chern_simons_density_matches_oracle():
