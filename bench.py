@@ -105,28 +105,40 @@ def profiled_traffic(kernel: str, form: str):
     return None
 
 
-def solve_latency(prec, p_i, p_f, seeds, tol, max_steps):
-    """Time-to-converged (max|g| < tol) from each seed's initial field, with the converged E and B; also
-    returns the converged fields."""
-    lat, steps, energy, baryon, conv, fields = [], [], [], [], [], []
+def solve_latency(p_i, p_f, seeds, tol, max_steps):
+    """Time-to-converged (max|g| < tol) of both forms from each seed's initial field, with the converged E
+    and B; and per seed the field rel-L2 of the FP32 solution against the FP64 one.  Each seed's field is
+    built once and solved in both forms."""
+    forms = (("fp32", _engine.FP32), ("fp64", _engine.FP64))
+    rec = {name: dict(lat=[], steps=[], energy=[], baryon=[], conv=[]) for name, _ in forms}
+    l2 = []
     for s in seeds:
         _, _, _, F = workload(seed=s)
-        eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
-        eng.upload(_engine.FIELD, F)
-        eng.sync()
-        t0 = time.perf_counter()
-        st = eng.anf_run(max_steps, tol, 64)
-        eng.sync()
-        lat.append(time.perf_counter() - t0)
-        steps.append(st.steps)
-        energy.append(st.energy)
-        baryon.append(st.baryon)
-        conv.append(bool(st.converged))
-        fields.append(eng.download(_engine.FIELD))
-        eng.close()
-    lat = np.array(lat)
-    return dict(tol=tol, seeds=len(seeds), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                steps=steps, converged=conv, energy=energy, baryon=baryon), fields
+        fields = {}
+        for name, prec in forms:
+            r = rec[name]
+            eng = _engine.Engine(_engine.THEORY_PION, prec, p_i, p_f)
+            eng.upload(_engine.FIELD, F)
+            eng.sync()
+            t0 = time.perf_counter()
+            st = eng.anf_run(max_steps, tol, 64)
+            eng.sync()
+            r["lat"].append(time.perf_counter() - t0)
+            r["steps"].append(st.steps)
+            r["energy"].append(st.energy)
+            r["baryon"].append(st.baryon)
+            r["conv"].append(bool(st.converged))
+            fields[name] = eng.download(_engine.FIELD)
+            eng.close()
+        l2.append(float(np.linalg.norm(fields["fp32"] - fields["fp64"]) / np.linalg.norm(fields["fp64"])))
+    out = {}
+    for name, _ in forms:
+        r = rec[name]
+        lat = np.array(r["lat"])
+        out[name] = dict(tol=tol, seeds=len(seeds), p50_s=float(np.percentile(lat, 50)),
+                         p99_s=float(np.percentile(lat, 99)), steps=r["steps"], converged=r["conv"],
+                         energy=r["energy"], baryon=r["baryon"])
+    return out, l2
 
 
 def host_identity():
@@ -332,7 +344,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=100,
                     help="steps per captured graph (capped at --steps; a remainder runs as direct launches)")
     ap.add_argument("--no-trace", action="store_true", help="time without the per-kernel HIP events")
-    ap.add_argument("--solve-seeds", type=int, default=3)
+    ap.add_argument("--solve-seeds", type=int, default=20, help="seeds of the solve-latency sample (0..n-1)")
     ap.add_argument("--solve-max-steps", type=int, default=20000)
     ap.add_argument("--tol", type=float, default=1.0e-4)
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps of the CPU baseline (~4.6 s each)")
@@ -418,24 +430,21 @@ def main():
                         bytes_per_launch=bytes_per_launch, measured_copy_GBs=copy_gbs,
                         frac_of_copy=achieved / copy_gbs)
 
-    latency, fields = {}, {}
+    latency, l2 = {}, []
     if args.solve_seeds > 0:
-        for name, prec in (("fp32", _engine.FP32), ("fp64", _engine.FP64)):
-            latency[name], fields[name] = solve_latency(prec, p_i, p_f, list(range(args.solve_seeds)), args.tol,
-                                                        args.solve_max_steps)
+        latency, l2 = solve_latency(p_i, p_f, list(range(args.solve_seeds)), args.tol, args.solve_max_steps)
+        for name in ("fp32", "fp64"):
             print(f"[bench] solve {name}: {latency[name]}", file=sys.stderr, flush=True)
 
     accuracy = None
     if "fp32" in latency and "fp64" in latency:
         e32, e64 = np.array(latency["fp32"]["energy"]), np.array(latency["fp64"]["energy"])
         b32, b64 = np.array(latency["fp32"]["baryon"]), np.array(latency["fp64"]["baryon"])
-        l2 = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(fields["fp32"], fields["fp64"])]
         accuracy = dict(max_rel_dE=float(np.max(np.abs(e32 - e64) / np.abs(e64))),
                         max_abs_dB=float(np.max(np.abs(b32 - b64))), max_field_rel_l2=max(l2),
                         converged=all(latency["fp32"]["converged"]) and all(latency["fp64"]["converged"]),
                         note="FP32 form vs FP64 form at convergence (max|g| < tol), same seeds; the FP64 form "
                              "matches the FP64 oracle step for step at this size")
-    fields.clear()
     c3 = c3_sample(args.c3_steps) if args.c3_steps > 0 else None
     if c3:
         print(f"[bench] c3: {c3}", file=sys.stderr, flush=True)
